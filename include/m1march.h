@@ -1,0 +1,182 @@
+/*
+ * m1march.h -- C ABI of the MI355X M1 energy march (libm1march.so).
+ *
+ * The library replaces the hot path of anis-el-hachmioui/proton-dose-m1:
+ * src/proton_solver/numerics/energy.py:1497-1644 (energy_march) and its
+ * callees (spatial.py:104-252, boundary.py:117-399, weno_z.py:13-321,
+ * flux.py:10-386, wave_speeds.py:12-105, realizability.py:132-215).
+ * Signatures use plain C types only: the Python drop-in binds them with
+ * ctypes (proton_solver/_native.py); any other FFI can bind them the same way.
+ *
+ * Arrays are C-ordered with z fastest, matching the reference's NumPy layout:
+ *   state U      : (4, nx, ny, nz) float64, components (phi, Jx, Jy, Jz)
+ *   dose         : (nx, ny, nz) float64 in Gy
+ *   material ids : (nx, ny, nz) int32, values in [0, n_materials)
+ *   inflow       : (nx, ny) float64 transverse fluence profile
+ *                  (boundary.py:68-80, BeamInflow.spatial)
+ *   boundary     : (4, nx, ny) float64 z-min inflow state
+ *
+ * Every function returns M1_OK (0) or a negative status; m1_last_error()
+ * gives the message.  The Python layer raises ValueError for M1_ERR_VALUE and
+ * NotImplementedError for M1_ERR_NOT_IMPLEMENTED, the exception types the
+ * reference raises in the same situations.  A context owns one HIP stream and
+ * its device buffers; it must be driven by one host thread at a time.
+ */
+#ifndef M1MARCH_H
+#define M1MARCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M1_ABI_VERSION 1
+
+enum {
+    M1_OK = 0,
+    M1_ERR_VALUE = -1,            /* invalid argument (ValueError)            */
+    M1_ERR_NOT_IMPLEMENTED = -2,  /* unsupported option (NotImplementedError) */
+    M1_ERR_HIP = -3,              /* HIP runtime error                        */
+    M1_ERR_NONFINITE = -4,        /* march produced a non-finite value        */
+    M1_ERR_STATE = -5             /* call order (no problem set, ...)         */
+};
+
+/* enums.py:22-38 (EnergyScheme subset the reference migrated, energy.py:1132-1165) */
+enum { M1_FORWARD_EULER = 0, M1_SSPRK3 = 1, M1_IF_SSPRK3 = 2 };
+/* enums.py:41-43 */
+enum { M1_WAVE_EXACT = 0, M1_WAVE_UNIT = 1 };
+/* enums.py:47-51 */
+enum { M1_SCATTERING_OFF = 0, M1_SCATTERING_ROSSI = 1, M1_SCATTERING_WENTZEL = 2 };
+/* enums.py:43-47 */
+enum { M1_INTERP_LINEAR = 0, M1_INTERP_PCHIP = 1 };
+/* Arithmetic of the march; no reference counterpart.  M1_FP64 keeps the
+ * reference's operation order with every operation separately rounded (the
+ * FP64 parity mode); M1_FP64_FAST lets the compiler fuse multiply-adds and
+ * multiplies by reciprocal constants; M1_FP32_BULK is the mixed-precision
+ * march. */
+enum { M1_FP64 = 0, M1_FP32_BULK = 1, M1_FP64_FAST = 2 };
+/* m1_step kinds: energy.py:663 forward_euler, 725 ssprk3, 878 if_ssprk3,
+ * 1194 strang_energy_step. */
+enum { M1_STEP_FORWARD_EULER = 0, M1_STEP_SSPRK3 = 1, M1_STEP_IF_SSPRK3 = 2,
+       M1_STEP_STRANG = 3 };
+
+typedef struct m1_ctx m1_ctx;
+
+/* One tabulated coefficient (data/<table>/<material>.csv): n strictly
+ * increasing energies [MeV], values, and PCHIP nodal slopes (required for the
+ * stopping power, stopping_power.py:57-138; used for the transport and removal
+ * tables only under M1_INTERP_PCHIP, may be NULL otherwise). */
+typedef struct {
+    int32_t n;
+    const double *energy;
+    const double *value;
+    const double *slope;
+} m1_table;
+
+/* One material: physics/stopping_power.py, scattering.py, nuclear.py tables
+ * and materials.py density / radiation length. */
+typedef struct {
+    m1_table stopping_power;   /* MeV/m */
+    m1_table transport;        /* Wentzel first transport coefficient, 1/m */
+    m1_table removal;          /* nuclear removal coefficient, 1/m */
+    double density;            /* kg/m^3 */
+    double radiation_length;   /* m */
+} m1_material;
+
+/* beam.py:6-13 fields the march reads (energy window and inflow spectrum). */
+typedef struct {
+    double energy;          /* MeV */
+    double energy_spread;   /* MeV, must be > 0 (boundary.py:85-88) */
+    double angular_spread;  /* rad */
+} m1_beam;
+
+/* config.py:16-41 fields the march reads, plus the precision knob. */
+typedef struct {
+    int32_t energy_scheme;              /* M1_FORWARD_EULER / M1_SSPRK3 / M1_IF_SSPRK3 */
+    int32_t wave_speeds;                /* M1_WAVE_* */
+    int32_t scattering;                 /* M1_SCATTERING_* */
+    int32_t nuclear;                    /* 0 / 1 */
+    int32_t coefficient_interpolation;  /* M1_INTERP_* */
+    int32_t precision;                  /* M1_FP64 / M1_FP32_BULK / M1_FP64_FAST */
+    double max_energy_step;             /* MeV */
+    double cfl;
+    double energy_min;                  /* MeV */
+    double energy_window_sigma;
+} m1_numerics;
+
+/* Library ABI version (M1_ABI_VERSION). */
+int m1_abi_version(void);
+
+/* Create a context on HIP device `device`.  No reference counterpart: the
+ * reference resolves its array module once per Solver (backend.py:12-59). */
+int m1_create(m1_ctx **out, int device);
+void m1_destroy(m1_ctx *ctx);
+
+/* Message of the last failing call on ctx (or of the last m1_create failure
+ * when ctx is NULL). */
+const char *m1_last_error(const m1_ctx *ctx);
+
+/* Bind a problem: grid (grid.py:19-42), material map (materials.py:65-127),
+ * material tables, inflow profile (boundary.py:68-80) and beam.  Uploads
+ * everything to device memory and zeroes the state and the dose.
+ * Replaces the argument marshalling of energy_march (energy.py:1497-1505). */
+int m1_set_problem(m1_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
+                   double dx, double dy, double dz,
+                   const int32_t *ids, int32_t n_materials,
+                   const m1_material *materials,
+                   const double *inflow_spatial, const m1_beam *beam);
+
+/* Device state U <- host (4,nx,ny,nz) array; NULL sets U = 0
+ * (solver.py:94-100). */
+int m1_set_state(m1_ctx *ctx, const double *U);
+int m1_get_state(m1_ctx *ctx, double *U);
+/* Dose accumulated by the last m1_march / m1_step calls, Gy. */
+int m1_get_dose(m1_ctx *ctx, double *dose);
+int m1_zero_dose(m1_ctx *ctx);
+
+/* energy_march (energy.py:1497-1644), static full domain: march the device
+ * state from E0 = E_beam + energy_window_sigma * sigma_E down to energy_min.
+ * The dose is reset, accumulated on the device and read with m1_get_dose; the
+ * final state stays on the device (m1_get_state).  energies receives E0 and
+ * the energy after every step (at most energies_cap values written; the full
+ * count goes to *n_energies).  Snapshots (energy.py:1533-1541, 1567-1578,
+ * 1626-1637): snapshots is n_snapshots*(4*nx*ny*nz) doubles, slot i filled
+ * when snapshot_energies[i] was reached (snapshot_taken[i] = 1). */
+int m1_march(m1_ctx *ctx, const m1_numerics *num,
+             double *energies, int32_t energies_cap, int32_t *n_energies,
+             const double *snapshot_energies, int32_t n_snapshots,
+             double *snapshots, int32_t *snapshot_taken);
+
+/* spatial_operator (spatial.py:104-252, active_domain=False) of the current
+ * device state at `energy`: rhs = -div F_HLL, (4,nx,ny,nz).  boundary_state
+ * is the z-min inflow state (4,nx,ny) or NULL for inflow.state(energy). */
+int m1_spatial_operator(m1_ctx *ctx, const m1_numerics *num, double energy,
+                        const double *boundary_state, double *rhs);
+
+/* compute_energy_step (energy.py:539-660, vacuum_relative_tolerance=None) of
+ * the current device state. */
+int m1_compute_energy_step(m1_ctx *ctx, const m1_numerics *num,
+                           double energy, double *dE);
+
+/* One integrator step of the current device state (kind = M1_STEP_*):
+ * forward_euler (energy.py:663-722), ssprk3 (725-873), if_ssprk3 (878-1130)
+ * or strang_energy_step (1194-1260).  The energy deposition of the step
+ * (MeV per m^3 per unit fluence, the `energy_deposition` array) is written to
+ * edep when it is not NULL; the device dose is not touched. */
+int m1_step(m1_ctx *ctx, const m1_numerics *num, int32_t kind,
+            double energy, double dE, double *edep);
+
+/* Kernel timing with HIP events on the context's stream (no reference
+ * counterpart; the reference times the whole march, solver.py:102-114). */
+int m1_set_profiling(m1_ctx *ctx, int32_t enable);
+int32_t m1_kernel_count(void);
+const char *m1_kernel_name(int32_t kernel);
+int m1_kernel_stats(m1_ctx *ctx, int32_t kernel, int64_t *launches,
+                    double *total_ms);
+int m1_reset_stats(m1_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+// Device-side data of the MI355X march: per-step coefficients, launch
+// arguments and the launchers of the kernels in m1_kernels.hip.
+//
+// m1_kernels.hip is compiled twice: with M1_EXACT=1 (namespace m1::exact,
+// every multiply and add rounded separately and every division kept, the
+// operation order of the reference's NumPy/Numba arithmetic) and with
+// M1_EXACT=0 (namespace m1::fast, fused multiply-adds and reciprocal
+// constants).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace m1 {
+
+constexpr int MAXMAT = 8;
+constexpr double MEV_TO_JOULE = 1.602176634e-13;  // numerics/energy.py:37
+
+// Scalars of one energy step, per material m (index = material id).
+// Energies: E (step start), E1 = E - dE, E2 = E - dE/2, E13 = E - dE/3,
+// E23 = E - 2 dE/3.  C(tau) = diag(exp(-min(tau_n,700)), exp(-min(tau_n+tau_s,700)) x3)
+// is the integrating factor of energy.py:273-310 ("p" = phi entry, "j" = J entries).
+struct StepCoef {
+    double dE;
+    double ed[4];            // inflow spectrum N(E) at E, E13, E23, E1 (boundary.py:90-99)
+    double rf;               // exp(-theta^2), boundary.py:102
+    double S0[MAXMAT];       // S(E)
+    double S1[MAXMAT];       // S(E1)
+    double S2[MAXMAT];       // S(E2)
+    double S13[MAXMAT];      // S(E13)
+    double S23[MAXMAT];      // S(E23)
+    double f1p[MAXMAT], f1j[MAXMAT];    // C(tau(dE))
+    double f2p[MAXMAT], f2j[MAXMAT];    // C(tau(dE/2))
+    double i1p[MAXMAT], i1j[MAXMAT];    // C(tau(dE))^-1
+    double i2p[MAXMAT], i2j[MAXMAT];    // C(tau(dE/2))^-1
+    double i13p[MAXMAT], i13j[MAXMAT];  // C(tau(dE/3))^-1
+    double i23p[MAXMAT], i23j[MAXMAT];  // C(tau(2dE/3))^-1
+    double rho[MAXMAT];                 // density, kg/m^3
+    double hn[MAXMAT], hs[MAXMAT];      // Strang half-step factors (energy.py:1168-1191)
+};
+
+struct GridDims {
+    int nx, ny, nz;
+    double dx, dy, dz;
+};
+
+enum StageKind : int {
+    STAGE0 = 0,    // V1 = rhs(U) dE + S0 U
+    STAGE1 = 1,    // V2 = 3/4 S0 U + 1/4 (C1^-1 rhs(U1) dE + V1)
+    STAGE2 = 2,    // U  = proj(C1 (1/3 S0 U + 2/3 (C2^-1 rhs(U2) dE + V2)) / S1), dose
+    STAGE_RHS = 3, // out = rhs(U)  (spatial_operator)
+    STAGE_FE = 4   // out = proj((rhs(U) dE + S0 U) / S1), edep = dE U0 S0
+};
+
+struct StageArgs {
+    GridDims g;
+    const double* src;      // stencil input: U (STAGE0/RHS/FE), V1 (STAGE1), V2 (STAGE2)
+    double* U;              // accepted state U^n; STAGE2 overwrites it with U^{n+1}
+    const double* V1;       // STAGE1: raw V1; STAGE2: raw V1 (phi only, for q1)
+    double* out;            // STAGE0: V1, STAGE1: V2, RHS: rhs, FE: U^{n+1}
+    double* dose;           // STAGE2 / FE: dose += edep * MEV / rho (nullable)
+    double* edep;           // STAGE2 / FE: per-cell energy deposition (nullable)
+    const uint8_t* ids;     // material ids
+    const double* bnd;      // (4, nx, ny) z-min boundary state of this stage
+    const StepCoef* coef;   // device copy of the step scalars
+    int wave_mode;          // 0 exact M1 speeds, 1 unit speeds
+};
+
+struct CflArgs {
+    GridDims g;
+    const double* U;
+    const uint8_t* ids;
+    const double* spatial;
+    const StepCoef* coef;   // uses S0, ed[0], rf
+    unsigned long long* out;  // max of the inverse step, as ordered bits
+    int wave_mode;
+};
+
+struct BoundaryArgs {
+    GridDims g;
+    const double* spatial;
+    const uint8_t* ids;
+    const StepCoef* coef;
+    double* bnd;            // 3 x (4, nx, ny): stage 0 (U_in(E)), stage 1, stage 2
+    int integrating_factor;
+};
+
+struct ScaleArgs {
+    long n;
+    double* U;
+    const uint8_t* ids;
+    const StepCoef* coef;
+    int nuclear, scattering;
+};
+
+#define M1_DECLARE_LAUNCHERS(NS)                                                          \
+    namespace NS {                                                                        \
+    hipError_t launch_stage(int kind, bool integrating_factor, const StageArgs& a,        \
+                            hipStream_t s);                                               \
+    hipError_t launch_cfl(const CflArgs& a, hipStream_t s);                               \
+    hipError_t launch_boundary(const BoundaryArgs& a, hipStream_t s);                     \
+    hipError_t launch_scale(const ScaleArgs& a, hipStream_t s);                           \
+    }
+
+M1_DECLARE_LAUNCHERS(exact)
+M1_DECLARE_LAUNCHERS(fast)
+
+}  // namespace m1

@@ -1,0 +1,888 @@
+// Host side of libm1march.so: the march driver of energy_march
+// (numerics/energy.py:1497-1644) and the C ABI declared in include/m1march.h.
+//
+// The driver keeps the reference's control flow: per step one CFL reduction on
+// the device, one 8-byte read-back of the inverse step, the step-local physics
+// scalars evaluated on the host in double (as the reference does in Python),
+// then the boundary-trace kernel and the three fused stage kernels on one HIP
+// stream.  State, stage variables and dose stay resident in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "m1_device.h"
+#include "m1_physics.h"
+#include "m1march.h"
+
+using m1::MaterialData;
+using m1::StepCoef;
+using m1::ValueError;
+
+namespace {
+
+enum KernelId {
+    K_STAGE0, K_STAGE1, K_STAGE2, K_RHS, K_FE, K_CFL, K_BOUNDARY, K_SCALE, K_COUNT
+};
+const char* const kKernelNames[K_COUNT] = {
+    "stage0", "stage1", "stage2", "spatial_operator", "forward_euler", "cfl", "boundary",
+    "source_half_step"};
+
+std::string g_create_error;
+
+struct HipError {
+    std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+struct NotImplemented {
+    std::string msg;
+};
+
+struct NonFinite {
+    std::string msg;
+};
+
+}  // namespace
+
+struct m1_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    bool have_problem = false;
+    int nx = 0, ny = 0, nz = 0;
+    long N = 0;
+    double dx = 0, dy = 0, dz = 0;
+    std::vector<MaterialData> mats;
+    double beam_energy = 0, energy_spread = 0, angular_spread = 0;
+
+    double *U = nullptr, *V1 = nullptr, *V2 = nullptr, *work = nullptr, *dose = nullptr,
+           *edep = nullptr, *bnd = nullptr, *spatial = nullptr;
+    uint8_t* ids = nullptr;
+    StepCoef* d_step = nullptr;
+    StepCoef* d_cfl = nullptr;
+    unsigned long long* d_red = nullptr;
+    StepCoef* h_step = nullptr;
+    StepCoef* h_cfl = nullptr;
+    unsigned long long* h_red = nullptr;
+
+    bool prof = false;
+    struct Pending {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Pending> pending;
+    int64_t launches[K_COUNT] = {};
+    double total_ms[K_COUNT] = {};
+
+    void free_problem();
+    ~m1_ctx();
+};
+
+void m1_ctx::free_problem()
+{
+    for (void* p : {(void*)U, (void*)V1, (void*)V2, (void*)work, (void*)dose, (void*)edep,
+                    (void*)bnd, (void*)spatial, (void*)ids})
+        if (p)
+            (void)hipFree(p);
+    U = V1 = V2 = work = dose = edep = bnd = spatial = nullptr;
+    ids = nullptr;
+    have_problem = false;
+}
+
+m1_ctx::~m1_ctx()
+{
+    (void)hipSetDevice(device);
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    free_problem();
+    for (void* p : {(void*)d_step, (void*)d_cfl, (void*)d_red})
+        if (p)
+            (void)hipFree(p);
+    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red})
+        if (p)
+            (void)hipHostFree(p);
+    for (auto& e : pool)
+        (void)hipEventDestroy(e);
+    for (auto& p : pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+// ------------------------------------------------------------- profiling
+hipEvent_t take_event(m1_ctx* c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+struct Timed {
+    m1_ctx* c;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(m1_ctx* ctx, int kernel) : c(ctx), k(kernel)
+    {
+        if (c->prof) {
+            a = take_event(c);
+            b = take_event(c);
+            hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+        }
+    }
+    ~Timed()
+    {
+        if (c->prof) {
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({k, a, b});
+        }
+    }
+};
+
+// Fold finished event pairs into the statistics (after a stream sync).
+void collect(m1_ctx* c)
+{
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->launches[p.kernel] += 1;
+            c->total_ms[p.kernel] += ms;
+        }
+        c->pool.push_back(p.a);
+        c->pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+void sync(m1_ctx* c)
+{
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (c->prof)
+        collect(c);
+}
+
+// ------------------------------------------------------------ numerics
+struct Num {
+    int scheme, wave, scattering, nuclear, interp, precision;
+    double max_step, cfl, emin, window;
+};
+
+Num read_numerics(const m1_numerics* n)
+{
+    if (!n)
+        throw ValueError{"numerics must not be NULL"};
+    Num r{n->energy_scheme, n->wave_speeds, n->scattering, n->nuclear,
+          n->coefficient_interpolation, n->precision, n->max_energy_step, n->cfl,
+          n->energy_min, n->energy_window_sigma};
+    if (r.scheme < M1_FORWARD_EULER || r.scheme > M1_IF_SSPRK3)
+        throw NotImplemented{"Energy scheme has not been migrated yet."};
+    if (r.wave != M1_WAVE_EXACT && r.wave != M1_WAVE_UNIT)
+        throw ValueError{"Unknown wave-speed mode."};
+    if (r.scattering < M1_SCATTERING_OFF || r.scattering > M1_SCATTERING_WENTZEL)
+        throw ValueError{"Unknown scattering model."};
+    if (r.interp != M1_INTERP_LINEAR && r.interp != M1_INTERP_PCHIP)
+        throw ValueError{"Unknown coefficient interpolation."};
+    if (r.precision == M1_FP32_BULK)
+        throw NotImplemented{"FP32_BULK precision is not available in this build."};
+    if (r.precision != M1_FP64 && r.precision != M1_FP64_FAST)
+        throw ValueError{"Unknown precision."};
+    if (!(std::isfinite(r.max_step) && std::isfinite(r.cfl) && std::isfinite(r.emin) &&
+          std::isfinite(r.window)))
+        throw ValueError{"Numerical parameters must be finite."};
+    if (r.max_step <= 0)
+        throw ValueError{"max_energy_step must be positive."};
+    if (!(r.cfl > 0 && r.cfl <= 1))
+        throw ValueError{"cfl must lie in (0, 1]."};
+    if (r.emin < 0)
+        throw ValueError{"energy_min cannot be negative."};
+    return r;
+}
+
+bool has_sources(const Num& n) { return n.scattering != M1_SCATTERING_OFF || n.nuclear; }
+
+m1::GridDims dims(const m1_ctx* c) { return {c->nx, c->ny, c->nz, c->dx, c->dy, c->dz}; }
+
+// Per-material optical depths over [E - d, E] (energy.py:234-270).
+void depths(const m1_ctx* c, const Num& n, double energy, double d, double* ts, double* tn)
+{
+    for (size_t m = 0; m < c->mats.size(); ++m) {
+        ts[m] = 0.0;
+        tn[m] = 0.0;
+        if (d <= 0.0)
+            continue;
+        if (n.scattering != M1_SCATTERING_OFF)
+            ts[m] = m1::scattering_depth(c->mats[m], n.scattering, n.interp, energy, d);
+        if (n.nuclear)
+            tn[m] = m1::nuclear_depth(c->mats[m], n.interp, energy, d);
+    }
+}
+
+// exp(+-min(tau_n, 700)), exp(+-min(tau_n + tau_s, 700)) (energy.py:294-308).
+void factors(size_t nm, const double* ts, const double* tn, bool inverse, double* fp, double* fj)
+{
+    const double sign = inverse ? 1.0 : -1.0;
+    for (size_t m = 0; m < nm; ++m) {
+        const double a = std::min(tn[m], 700.0);
+        const double b = std::min(tn[m] + ts[m], 700.0);
+        fp[m] = std::exp(sign * a);
+        fj[m] = std::exp(sign * b);
+    }
+}
+
+// All scalars of one step at (energy, dE).
+void fill_step(const m1_ctx* c, const Num& n, double energy, double dE, bool ifac, bool strang,
+               StepCoef& s)
+{
+    std::memset(&s, 0, sizeof s);
+    const size_t nm = c->mats.size();
+    s.dE = dE;
+    const double e13 = energy - dE / 3.0, e23 = energy - 2.0 * dE / 3.0, e1 = energy - dE,
+                 e2 = energy - 0.5 * dE;
+    s.ed[0] = m1::energy_density(energy, c->beam_energy, c->energy_spread);
+    s.ed[1] = m1::energy_density(e13, c->beam_energy, c->energy_spread);
+    s.ed[2] = m1::energy_density(e23, c->beam_energy, c->energy_spread);
+    s.ed[3] = m1::energy_density(e1, c->beam_energy, c->energy_spread);
+    s.rf = std::exp(-(c->angular_spread * c->angular_spread));
+    for (size_t m = 0; m < nm; ++m) {
+        const MaterialData& md = c->mats[m];
+        s.S0[m] = m1::stopping_power(md, energy);
+        s.S1[m] = m1::stopping_power(md, e1);
+        s.S2[m] = m1::stopping_power(md, e2);
+        s.S13[m] = m1::stopping_power(md, e13);
+        s.S23[m] = m1::stopping_power(md, e23);
+        s.rho[m] = md.density;
+        s.f1p[m] = s.f1j[m] = s.f2p[m] = s.f2j[m] = 1.0;
+        s.i1p[m] = s.i1j[m] = s.i2p[m] = s.i2j[m] = 1.0;
+        s.i13p[m] = s.i13j[m] = s.i23p[m] = s.i23j[m] = 1.0;
+        s.hn[m] = s.hs[m] = 1.0;
+    }
+    if (ifac) {
+        double ts[4][m1::MAXMAT], tn[4][m1::MAXMAT];
+        const double ds[4] = {dE / 3.0, 0.5 * dE, 2.0 * dE / 3.0, dE};
+        for (int q = 0; q < 4; ++q)
+            depths(c, n, energy, ds[q], ts[q], tn[q]);
+        factors(nm, ts[3], tn[3], false, s.f1p, s.f1j);
+        factors(nm, ts[1], tn[1], false, s.f2p, s.f2j);
+        factors(nm, ts[3], tn[3], true, s.i1p, s.i1j);
+        factors(nm, ts[1], tn[1], true, s.i2p, s.i2j);
+        factors(nm, ts[0], tn[0], true, s.i13p, s.i13j);
+        factors(nm, ts[2], tn[2], true, s.i23p, s.i23j);
+    }
+    if (strang) {
+        // energy.py:1206-1227 with the half-step factors of 1168-1191.
+        for (size_t m = 0; m < nm; ++m) {
+            double eta = 0.0, taun = 0.0;
+            if (n.scattering != M1_SCATTERING_OFF)
+                eta = m1::scattering_depth(c->mats[m], n.scattering, n.interp, energy, dE);
+            if (n.nuclear)
+                taun = m1::nuclear_depth(c->mats[m], n.interp, energy, dE);
+            s.hn[m] = std::exp(-std::min(0.5 * taun, 700.0));
+            s.hs[m] = std::exp(-std::min(0.5 * eta, 700.0));
+        }
+    }
+}
+
+hipError_t stage(const m1_ctx* c, const Num& n, int kind, bool ifac, const m1::StageArgs& a)
+{
+    if (n.precision == M1_FP64)
+        return m1::exact::launch_stage(kind, ifac, a, c->stream);
+    return m1::fast::launch_stage(kind, ifac, a, c->stream);
+}
+
+m1::StageArgs base_args(const m1_ctx* c, const Num& n)
+{
+    m1::StageArgs a{};
+    a.g = dims(c);
+    a.ids = c->ids;
+    a.coef = c->d_step;
+    a.wave_mode = n.wave;
+    return a;
+}
+
+void upload_step(m1_ctx* c)
+{
+    hip_check(hipMemcpyAsync(c->d_step, c->h_step, sizeof(StepCoef), hipMemcpyHostToDevice,
+                             c->stream),
+              "coefficient upload");
+}
+
+void run_boundary(m1_ctx* c, const Num& n, bool ifac)
+{
+    m1::BoundaryArgs b{dims(c), c->spatial, c->ids, c->d_step, c->bnd, ifac ? 1 : 0};
+    Timed t(c, K_BOUNDARY);
+    hipError_t e = n.precision == M1_FP64 ? m1::exact::launch_boundary(b, c->stream)
+                                                 : m1::fast::launch_boundary(b, c->stream);
+    hip_check(e, "boundary kernel");
+}
+
+void run_scale(m1_ctx* c, const Num& n)
+{
+    m1::ScaleArgs s{c->N, c->U, c->ids, c->d_step, n.nuclear ? 1 : 0,
+                    n.scattering != M1_SCATTERING_OFF ? 1 : 0};
+    Timed t(c, K_SCALE);
+    hip_check(m1::fast::launch_scale(s, c->stream), "source half-step kernel");
+}
+
+// Three fused SSPRK3 stages (energy.py:725-873 / 878-1130).  Requires the
+// step coefficients on the device.
+void rk3(m1_ctx* c, const Num& n, bool ifac, double* dose, double* edep)
+{
+    run_boundary(c, n, ifac);
+    const long m2 = long(c->nx) * c->ny;
+    m1::StageArgs a = base_args(c, n);
+    a.U = c->U;
+    a.V1 = c->V1;
+    {
+        a.src = c->U;
+        a.out = c->V1;
+        a.bnd = c->bnd;
+        Timed t(c, K_STAGE0);
+        hip_check(stage(c, n, m1::STAGE0, false, a), "stage 0 kernel");
+    }
+    {
+        a.src = c->V1;
+        a.out = c->V2;
+        a.bnd = c->bnd + 4 * m2;
+        Timed t(c, K_STAGE1);
+        hip_check(stage(c, n, m1::STAGE1, ifac, a), "stage 1 kernel");
+    }
+    {
+        a.src = c->V2;
+        a.out = nullptr;
+        a.dose = dose;
+        a.edep = edep;
+        a.bnd = c->bnd + 8 * m2;
+        Timed t(c, K_STAGE2);
+        hip_check(stage(c, n, m1::STAGE2, ifac, a), "stage 2 kernel");
+    }
+}
+
+// forward_euler (energy.py:663-722): new state into `work`, then swap.
+void forward_euler(m1_ctx* c, const Num& n, double* dose, double* edep)
+{
+    run_boundary(c, n, false);
+    m1::StageArgs a = base_args(c, n);
+    a.src = c->U;
+    a.U = c->U;
+    a.out = c->work;
+    a.dose = dose;
+    a.edep = edep;
+    a.bnd = c->bnd;
+    {
+        Timed t(c, K_FE);
+        hip_check(stage(c, n, m1::STAGE_FE, false, a), "forward Euler kernel");
+    }
+    std::swap(c->U, c->work);
+}
+
+// advance_energy_step (energy.py:1132-1165).
+void advance(m1_ctx* c, const Num& n, double* dose, double* edep)
+{
+    if (n.scheme == M1_FORWARD_EULER)
+        forward_euler(c, n, dose, edep);
+    else
+        rk3(c, n, false, dose, edep);
+}
+
+// One step of the requested kind; coefficients are filled and uploaded here.
+void step(m1_ctx* c, const Num& n, int kind, double energy, double dE, double* dose,
+          double* edep)
+{
+    if (dE <= 0.0 && kind != M1_STEP_FORWARD_EULER)
+        throw ValueError{"dE must be positive."};
+    const bool ifac = kind == M1_STEP_IF_SSPRK3 && has_sources(n);
+    const bool strang = kind == M1_STEP_STRANG;
+    fill_step(c, n, energy, dE, ifac, strang, *c->h_step);
+    upload_step(c);
+    switch (kind) {
+    case M1_STEP_FORWARD_EULER:
+        forward_euler(c, n, dose, edep);
+        break;
+    case M1_STEP_SSPRK3:
+        rk3(c, n, false, dose, edep);
+        break;
+    case M1_STEP_IF_SSPRK3:
+        rk3(c, n, ifac, dose, edep);
+        break;
+    case M1_STEP_STRANG:
+        if (has_sources(n))
+            run_scale(c, n);
+        advance(c, n, dose, edep);
+        if (has_sources(n))
+            run_scale(c, n);
+        break;
+    default:
+        throw ValueError{"unknown step kind"};
+    }
+}
+
+// compute_energy_step (energy.py:539-660) of the device state at `energy`.
+double cfl_step(m1_ctx* c, const Num& n, double energy)
+{
+    StepCoef& s = *c->h_cfl;
+    std::memset(&s, 0, sizeof s);
+    for (size_t m = 0; m < c->mats.size(); ++m)
+        s.S0[m] = m1::stopping_power(c->mats[m], energy);
+    s.ed[0] = m1::energy_density(energy, c->beam_energy, c->energy_spread);
+    s.rf = std::exp(-(c->angular_spread * c->angular_spread));
+    hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, sizeof(StepCoef), hipMemcpyHostToDevice,
+                             c->stream),
+              "CFL coefficient upload");
+    hip_check(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream),
+              "CFL reset");
+    m1::CflArgs a{dims(c), c->U, c->ids, c->spatial, c->d_cfl, c->d_red, n.wave};
+    {
+        Timed t(c, K_CFL);
+        hipError_t e = n.precision == M1_FP64 ? m1::exact::launch_cfl(a, c->stream)
+                                                     : m1::fast::launch_cfl(a, c->stream);
+        hip_check(e, "CFL kernel");
+    }
+    hip_check(hipMemcpyAsync(c->h_red, c->d_red, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream),
+              "CFL read-back");
+    sync(c);
+    double mx;
+    std::memcpy(&mx, c->h_red, sizeof mx);
+    if (!std::isfinite(mx))
+        throw NonFinite{"non-finite M1 state detected by the CFL reduction"};
+    const double dE = n.cfl / mx;
+    return dE <= n.max_step ? dE : n.max_step;
+}
+
+void require_problem(const m1_ctx* c)
+{
+    if (!c->have_problem)
+        throw ValueError{"no problem bound: call m1_set_problem first"};
+}
+
+MaterialData read_material(const m1_material& m, int idx)
+{
+    auto table = [&](const m1_table& t, bool need_slope, const char* what) {
+        m1::Table r;
+        if (t.n < 2 || !t.energy || !t.value)
+            throw ValueError{std::string(what) + " table of material " + std::to_string(idx) +
+                             " must contain at least two points."};
+        r.E.assign(t.energy, t.energy + t.n);
+        r.v.assign(t.value, t.value + t.n);
+        for (int i = 1; i < t.n; ++i)
+            if (!(r.E[i] > r.E[i - 1]))
+                throw ValueError{std::string(what) + " energies must be strictly increasing."};
+        if (t.slope)
+            r.slope.assign(t.slope, t.slope + t.n);
+        else if (need_slope)
+            throw ValueError{std::string(what) + " table needs PCHIP slopes."};
+        return r;
+    };
+    MaterialData d;
+    d.sp = table(m.stopping_power, true, "Stopping-power");
+    for (double v : d.sp.v)
+        if (!(std::isfinite(v) && v > 0.0))
+            throw ValueError{"Stopping-power values must be finite and positive."};
+    d.tr = table(m.transport, false, "Transport");
+    d.nu = table(m.removal, false, "Nuclear-attenuation");
+    d.density = m.density;
+    d.radiation_length = m.radiation_length;
+    if (!(d.density > 0.0 && std::isfinite(d.density)))
+        throw ValueError{"material density must be finite and positive"};
+    return d;
+}
+
+template <class F>
+int guarded(m1_ctx* c, F&& f)
+{
+    if (!c)
+        return M1_ERR_STATE;
+    try {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        f();
+        c->err.clear();
+        return M1_OK;
+    } catch (const ValueError& e) {
+        c->err = e.msg;
+        return M1_ERR_VALUE;
+    } catch (const NotImplemented& e) {
+        c->err = e.msg;
+        return M1_ERR_NOT_IMPLEMENTED;
+    } catch (const NonFinite& e) {
+        c->err = e.msg;
+        return M1_ERR_NONFINITE;
+    } catch (const HipError& e) {
+        c->err = e.msg;
+        return M1_ERR_HIP;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return M1_ERR_HIP;
+    }
+}
+
+template <class T>
+void dev_alloc(T** p, size_t count, const char* what)
+{
+    hip_check(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)), what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int m1_abi_version(void) { return M1_ABI_VERSION; }
+
+int m1_create(m1_ctx** out, int device)
+{
+    if (!out) {
+        g_create_error = "out must not be NULL";
+        return M1_ERR_VALUE;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        g_create_error = "no HIP device is available";
+        return M1_ERR_HIP;
+    }
+    if (device < 0 || device >= count) {
+        g_create_error = "device ordinal out of range";
+        return M1_ERR_VALUE;
+    }
+    m1_ctx* c = new m1_ctx;
+    c->device = device;
+    try {
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        dev_alloc(&c->d_step, 1, "hipMalloc coefficients");
+        dev_alloc(&c->d_cfl, 1, "hipMalloc coefficients");
+        dev_alloc(&c->d_red, 1, "hipMalloc reduction");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_step), sizeof(StepCoef)),
+                  "hipHostMalloc");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_cfl), sizeof(StepCoef)),
+                  "hipHostMalloc");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_red), sizeof(unsigned long long)),
+                  "hipHostMalloc");
+    } catch (const HipError& e) {
+        g_create_error = e.msg;
+        delete c;
+        return M1_ERR_HIP;
+    }
+    *out = c;
+    return M1_OK;
+}
+
+void m1_destroy(m1_ctx* ctx) { delete ctx; }
+
+const char* m1_last_error(const m1_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int m1_set_problem(m1_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double dx, double dy,
+                   double dz, const int32_t* ids, int32_t n_materials,
+                   const m1_material* materials, const double* inflow_spatial,
+                   const m1_beam* beam)
+{
+    return guarded(ctx, [&] {
+        if (nx <= 0 || ny <= 0 || nz <= 0)
+            throw ValueError{"Geometry resolution must contain positive integers."};
+        if (nz < 4)
+            throw ValueError{"WENO-Z inflow extrapolation requires at least four z cells."};
+        if (!(dx > 0 && dy > 0 && dz > 0))
+            throw ValueError{"cell sizes must be positive"};
+        if (n_materials < 1 || n_materials > m1::MAXMAT)
+            throw ValueError{"between 1 and 8 materials are supported"};
+        if (!ids || !materials || !inflow_spatial || !beam)
+            throw ValueError{"ids, materials, inflow and beam must not be NULL"};
+        if (beam->energy_spread == 0.0)
+            throw ValueError{"Zero energy spread is not supported by the Gaussian inflow."};
+        const long N = long(nx) * ny * nz;
+        std::vector<uint8_t> id8(static_cast<size_t>(N));
+        for (long i = 0; i < N; ++i) {
+            if (ids[i] < 0 || ids[i] >= n_materials)
+                throw ValueError{"material id out of range"};
+            id8[size_t(i)] = uint8_t(ids[i]);
+        }
+        std::vector<MaterialData> mats;
+        for (int m = 0; m < n_materials; ++m)
+            mats.push_back(read_material(materials[m], m));
+
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->free_problem();
+        ctx->nx = nx;
+        ctx->ny = ny;
+        ctx->nz = nz;
+        ctx->N = N;
+        ctx->dx = dx;
+        ctx->dy = dy;
+        ctx->dz = dz;
+        ctx->mats = std::move(mats);
+        ctx->beam_energy = beam->energy;
+        ctx->energy_spread = beam->energy_spread;
+        ctx->angular_spread = beam->angular_spread;
+        const size_t m2 = size_t(nx) * ny;
+        dev_alloc(&ctx->U, 4 * size_t(N), "hipMalloc state");
+        dev_alloc(&ctx->V1, 4 * size_t(N), "hipMalloc stage 1");
+        dev_alloc(&ctx->V2, 4 * size_t(N), "hipMalloc stage 2");
+        dev_alloc(&ctx->work, 4 * size_t(N), "hipMalloc work");
+        dev_alloc(&ctx->dose, size_t(N), "hipMalloc dose");
+        dev_alloc(&ctx->edep, size_t(N), "hipMalloc deposition");
+        dev_alloc(&ctx->bnd, 12 * m2, "hipMalloc boundary");
+        dev_alloc(&ctx->spatial, m2, "hipMalloc inflow");
+        dev_alloc(&ctx->ids, size_t(N), "hipMalloc ids");
+        hip_check(hipMemcpy(ctx->ids, id8.data(), size_t(N), hipMemcpyHostToDevice), "ids upload");
+        hip_check(hipMemcpy(ctx->spatial, inflow_spatial, m2 * sizeof(double),
+                            hipMemcpyHostToDevice),
+                  "inflow upload");
+        hip_check(hipMemsetAsync(ctx->U, 0, 4 * size_t(N) * sizeof(double), ctx->stream),
+                  "state reset");
+        hip_check(hipMemsetAsync(ctx->dose, 0, size_t(N) * sizeof(double), ctx->stream),
+                  "dose reset");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->have_problem = true;
+    });
+}
+
+int m1_set_state(m1_ctx* ctx, const double* U)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const size_t bytes = 4 * size_t(ctx->N) * sizeof(double);
+        if (U)
+            hip_check(hipMemcpyAsync(ctx->U, U, bytes, hipMemcpyHostToDevice, ctx->stream),
+                      "state upload");
+        else
+            hip_check(hipMemsetAsync(ctx->U, 0, bytes, ctx->stream), "state reset");
+        sync(ctx);
+    });
+}
+
+int m1_get_state(m1_ctx* ctx, double* U)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        if (!U)
+            throw ValueError{"U must not be NULL"};
+        hip_check(hipMemcpyAsync(U, ctx->U, 4 * size_t(ctx->N) * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream),
+                  "state download");
+        sync(ctx);
+    });
+}
+
+int m1_get_dose(m1_ctx* ctx, double* dose)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        if (!dose)
+            throw ValueError{"dose must not be NULL"};
+        hip_check(hipMemcpyAsync(dose, ctx->dose, size_t(ctx->N) * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream),
+                  "dose download");
+        sync(ctx);
+    });
+}
+
+int m1_zero_dose(m1_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        hip_check(hipMemsetAsync(ctx->dose, 0, size_t(ctx->N) * sizeof(double), ctx->stream),
+                  "dose reset");
+        sync(ctx);
+    });
+}
+
+int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t energies_cap,
+             int32_t* n_energies, const double* snapshot_energies, int32_t n_snapshots,
+             double* snapshots, int32_t* snapshot_taken)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const Num n = read_numerics(num);
+        if (n_snapshots < 0 || (n_snapshots > 0 && !snapshot_energies))
+            throw ValueError{"invalid snapshot request"};
+        double energy = ctx->beam_energy + n.window * ctx->energy_spread;
+        const double emin = n.emin;
+        // Unique requested energies inside [emin, E0], descending (energy.py:1533-1541).
+        std::vector<double> targets;
+        for (int i = 0; i < n_snapshots; ++i) {
+            if (snapshot_taken)
+                snapshot_taken[i] = 0;
+            const double t = snapshot_energies[i];
+            if (emin <= t && t <= energy &&
+                std::find(targets.begin(), targets.end(), t) == targets.end())
+                targets.push_back(t);
+        }
+        std::sort(targets.begin(), targets.end(), std::greater<double>());
+        hip_check(hipMemsetAsync(ctx->dose, 0, size_t(ctx->N) * sizeof(double), ctx->stream),
+                  "dose reset");
+        int32_t ne = 0;
+        auto record = [&](double e) {
+            if (energies && ne < energies_cap)
+                energies[ne] = e;
+            ++ne;
+        };
+        record(energy);
+        size_t ti = 0;
+        const int kind = n.scheme == M1_IF_SSPRK3 ? M1_STEP_IF_SSPRK3
+                         : has_sources(n)         ? M1_STEP_STRANG
+                         : n.scheme == M1_FORWARD_EULER ? M1_STEP_FORWARD_EULER
+                                                        : M1_STEP_SSPRK3;
+        while (energy > emin) {
+            double dE = cfl_step(ctx, n, energy);
+            if (ti < targets.size()) {
+                const double t = targets[ti];
+                if (energy > t && t >= energy - dE)
+                    dE = energy - t;
+            }
+            if (energy - emin < dE)
+                dE = energy - emin;
+            step(ctx, n, kind, energy, dE, ctx->dose, nullptr);
+            energy -= dE;
+            record(energy);
+            if (ti < targets.size()) {
+                const double t = targets[ti];
+                if (std::fabs(energy - t) <= 1e-12) {
+                    if (snapshots) {
+                        // slot of the first request equal to this target
+                        int slot = 0;
+                        for (int i = 0; i < n_snapshots; ++i)
+                            if (snapshot_energies[i] == t) {
+                                slot = i;
+                                break;
+                            }
+                        hip_check(hipMemcpyAsync(snapshots + size_t(slot) * 4 * size_t(ctx->N),
+                                                 ctx->U, 4 * size_t(ctx->N) * sizeof(double),
+                                                 hipMemcpyDeviceToHost, ctx->stream),
+                                  "snapshot download");
+                        if (snapshot_taken)
+                            snapshot_taken[slot] = 1;
+                    }
+                    ++ti;
+                }
+            }
+        }
+        sync(ctx);
+        if (n_energies)
+            *n_energies = ne;
+    });
+}
+
+int m1_spatial_operator(m1_ctx* ctx, const m1_numerics* num, double energy,
+                        const double* boundary_state, double* rhs)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const Num n = read_numerics(num);
+        if (!rhs)
+            throw ValueError{"rhs must not be NULL"};
+        const size_t m2 = size_t(ctx->nx) * ctx->ny;
+        if (boundary_state) {
+            hip_check(hipMemcpyAsync(ctx->bnd, boundary_state, 4 * m2 * sizeof(double),
+                                     hipMemcpyHostToDevice, ctx->stream),
+                      "boundary upload");
+            fill_step(ctx, n, energy, 0.0, false, false, *ctx->h_step);
+        } else {
+            fill_step(ctx, n, energy, 0.0, false, false, *ctx->h_step);
+        }
+        upload_step(ctx);
+        if (!boundary_state)
+            run_boundary(ctx, n, false);
+        m1::StageArgs a = base_args(ctx, n);
+        a.src = ctx->U;
+        a.U = ctx->U;
+        a.out = ctx->work;
+        a.bnd = ctx->bnd;
+        {
+            Timed t(ctx, K_RHS);
+            hip_check(stage(ctx, n, m1::STAGE_RHS, false, a), "spatial operator kernel");
+        }
+        hip_check(hipMemcpyAsync(rhs, ctx->work, 4 * size_t(ctx->N) * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream),
+                  "rhs download");
+        sync(ctx);
+    });
+}
+
+int m1_compute_energy_step(m1_ctx* ctx, const m1_numerics* num, double energy, double* dE)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const Num n = read_numerics(num);
+        const double d = cfl_step(ctx, n, energy);
+        if (dE)
+            *dE = d;
+    });
+}
+
+int m1_step(m1_ctx* ctx, const m1_numerics* num, int32_t kind, double energy, double dE,
+            double* edep)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const Num n = read_numerics(num);
+        step(ctx, n, kind, energy, dE, nullptr, edep ? ctx->edep : nullptr);
+        if (edep)
+            hip_check(hipMemcpyAsync(edep, ctx->edep, size_t(ctx->N) * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->stream),
+                      "deposition download");
+        sync(ctx);
+    });
+}
+
+int m1_set_profiling(m1_ctx* ctx, int32_t enable)
+{
+    return guarded(ctx, [&] {
+        sync(ctx);
+        ctx->prof = enable != 0;
+    });
+}
+
+int32_t m1_kernel_count(void) { return K_COUNT; }
+
+const char* m1_kernel_name(int32_t kernel)
+{
+    return kernel >= 0 && kernel < K_COUNT ? kKernelNames[kernel] : "";
+}
+
+int m1_kernel_stats(m1_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms)
+{
+    return guarded(ctx, [&] {
+        if (kernel < 0 || kernel >= K_COUNT)
+            throw ValueError{"kernel index out of range"};
+        sync(ctx);
+        if (launches)
+            *launches = ctx->launches[kernel];
+        if (total_ms)
+            *total_ms = ctx->total_ms[kernel];
+    });
+}
+
+int m1_reset_stats(m1_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        sync(ctx);
+        for (int k = 0; k < K_COUNT; ++k) {
+            ctx->launches[k] = 0;
+            ctx->total_ms[k] = 0.0;
+        }
+    });
+}
+
+}  // extern "C"

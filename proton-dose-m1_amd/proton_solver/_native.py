@@ -1,0 +1,303 @@
+"""ctypes binding of libm1march.so (declarations: include/m1march.h).
+
+The library is the only compute path of this package: there is no CPU
+fallback.  Loading fails loudly when the library is missing or was built for
+another ABI version.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .enums import CoefficientInterpolation, EnergyScheme, Precision, Scattering, WaveSpeeds
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libm1march.so"
+ABI_VERSION = 1
+
+M1_OK = 0
+M1_ERR_VALUE = -1
+M1_ERR_NOT_IMPLEMENTED = -2
+M1_ERR_HIP = -3
+M1_ERR_NONFINITE = -4
+M1_ERR_STATE = -5
+
+STEP_FORWARD_EULER = 0
+STEP_SSPRK3 = 1
+STEP_IF_SSPRK3 = 2
+STEP_STRANG = 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class m1_table(C.Structure):
+    _fields_ = [("n", C.c_int32), ("energy", _dp), ("value", _dp), ("slope", _dp)]
+
+
+class m1_material(C.Structure):
+    _fields_ = [("stopping_power", m1_table), ("transport", m1_table), ("removal", m1_table),
+                ("density", C.c_double), ("radiation_length", C.c_double)]
+
+
+class m1_beam(C.Structure):
+    _fields_ = [("energy", C.c_double), ("energy_spread", C.c_double),
+                ("angular_spread", C.c_double)]
+
+
+class m1_numerics(C.Structure):
+    _fields_ = [("energy_scheme", C.c_int32), ("wave_speeds", C.c_int32),
+                ("scattering", C.c_int32), ("nuclear", C.c_int32),
+                ("coefficient_interpolation", C.c_int32), ("precision", C.c_int32),
+                ("max_energy_step", C.c_double), ("cfl", C.c_double),
+                ("energy_min", C.c_double), ("energy_window_sigma", C.c_double)]
+
+
+# (name, restype, argtypes) of every exported function.
+SIGNATURES = [
+    ("m1_abi_version", C.c_int, []),
+    ("m1_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("m1_destroy", None, [C.c_void_p]),
+    ("m1_last_error", C.c_char_p, [C.c_void_p]),
+    ("m1_set_problem", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                 C.c_double, C.c_double, _ip, C.c_int32,
+                                 C.POINTER(m1_material), _dp, C.POINTER(m1_beam)]),
+    ("m1_set_state", C.c_int, [C.c_void_p, _dp]),
+    ("m1_get_state", C.c_int, [C.c_void_p, _dp]),
+    ("m1_get_dose", C.c_int, [C.c_void_p, _dp]),
+    ("m1_zero_dose", C.c_int, [C.c_void_p]),
+    ("m1_march", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), _dp, C.c_int32,
+                           C.POINTER(C.c_int32), _dp, C.c_int32, _dp, _ip]),
+    ("m1_spatial_operator", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_double, _dp, _dp]),
+    ("m1_compute_energy_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_double, _dp]),
+    ("m1_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_int32, C.c_double,
+                          C.c_double, _dp]),
+    ("m1_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_kernel_count", C.c_int32, []),
+    ("m1_kernel_name", C.c_char_p, [C.c_int32]),
+    ("m1_kernel_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_double)]),
+    ("m1_reset_stats", C.c_int, [C.c_void_p]),
+]
+
+_lib = None
+
+
+def load_library(path: os.PathLike | None = None):
+    """Load libm1march.so and declare its signatures (no GPU call is made)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: the MI355X march library must be built first "
+            "(python proton-dose-m1_amd/build_native.py)."
+        )
+    lib = C.CDLL(str(p))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.m1_abi_version() != ABI_VERSION:
+        raise RuntimeError("libm1march.so ABI version mismatch; rebuild the library.")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    return a.ctypes.data_as(_dp)
+
+
+_SCHEME = {EnergyScheme.FORWARD_EULER: 0, EnergyScheme.SSPRK3: 1, EnergyScheme.IF_SSPRK3: 2}
+_WAVE = {WaveSpeeds.EXACT: 0, WaveSpeeds.UNIT: 1}
+_SCAT = {Scattering.OFF: 0, Scattering.ROSSI: 1, Scattering.WENTZEL: 2}
+_INTERP = {CoefficientInterpolation.LINEAR: 0, CoefficientInterpolation.PCHIP: 1}
+PRECISION_CODE = {Precision.FP64: 0, Precision.FP32_BULK: 1, Precision.FP64_FAST: 2}
+FP64 = 0
+FP32_BULK = 1
+FP64_FAST = 2
+
+
+def numerics_struct(config, precision_code: int | None = None) -> m1_numerics:
+    """m1_numerics of a SolverConfig (config.py fields the march reads)."""
+    n = config.numerics
+    ph = config.physics
+    if n.energy_scheme not in _SCHEME:
+        raise NotImplementedError(f"Energy scheme {n.energy_scheme} has not been migrated yet.")
+    if precision_code is None:
+        precision_code = PRECISION_CODE[config.device.precision]
+    return m1_numerics(_SCHEME[n.energy_scheme], _WAVE[n.wave_speeds], _SCAT[ph.scattering],
+                       int(bool(ph.nuclear)), _INTERP[ph.coefficient_interpolation],
+                       int(precision_code), float(n.max_energy_step), float(n.cfl),
+                       float(n.energy_min), float(n.energy_window_sigma))
+
+
+def _table(E, v, slope, keep):
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    s = None if slope is None else np.ascontiguousarray(slope, dtype=np.float64)
+    keep += [E, v, s]
+    return m1_table(len(E), _ptr(E), _ptr(v), _ptr(s))
+
+
+def material_struct(material, keep: list) -> m1_material:
+    from .physics.materials import density, radiation_length
+    from .physics.nuclear import load_nuclear_table, nuclear_table_slopes
+    from .physics.scattering import load_transport_table, transport_table_slopes
+    from .physics.stopping_power import stopping_power_interpolant
+
+    E, v, s = stopping_power_interpolant(material)
+    sp = _table(E, v, s, keep)
+    tr = _table(*load_transport_table(material), transport_table_slopes(material), keep)
+    nu = _table(*load_nuclear_table(material), nuclear_table_slopes(material), keep)
+    return m1_material(sp, tr, nu, density(material), radiation_length(material))
+
+
+class MarchError(RuntimeError):
+    """HIP runtime failure or non-finite state reported by the library."""
+
+
+class Context:
+    """One libm1march context (device buffers + stream) bound to a problem."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.m1_create(C.byref(h), int(device))
+        if rc != M1_OK:
+            raise MarchError(self.lib.m1_last_error(None).decode())
+        self.h = h
+        self.shape = None
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.m1_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc == M1_OK:
+            return
+        msg = self.lib.m1_last_error(self.h).decode()
+        if rc == M1_ERR_VALUE:
+            raise ValueError(msg)
+        if rc == M1_ERR_NOT_IMPLEMENTED:
+            raise NotImplementedError(msg)
+        raise MarchError(msg)
+
+    # --- problem -------------------------------------------------------
+    def set_problem(self, grid, material_map, inflow, beam):
+        nx, ny, nz = (int(s) for s in grid.shape)
+        ids = np.ascontiguousarray(np.asarray(material_map.ids), dtype=np.int32)
+        if ids.shape != (nx, ny, nz):
+            raise ValueError("material ids must have the grid shape")
+        spatial = np.ascontiguousarray(np.asarray(inflow.spatial), dtype=np.float64)
+        if spatial.shape != (nx, ny):
+            raise ValueError("inflow profile must have shape (nx, ny)")
+        keep = []
+        mats = (m1_material * len(material_map.materials))()
+        for i, m in enumerate(material_map.materials):
+            mats[i] = material_struct(m, keep)
+        b = m1_beam(float(beam.energy), float(beam.energy_spread), float(beam.angular_spread))
+        self._check(self.lib.m1_set_problem(self.h, nx, ny, nz, float(grid.dx), float(grid.dy),
+                                            float(grid.dz), _ptr(ids), len(mats), mats,
+                                            _ptr(spatial), C.byref(b)))
+        self.shape = (nx, ny, nz)
+
+    @property
+    def n(self):
+        nx, ny, nz = self.shape
+        return nx * ny * nz
+
+    def set_state(self, U=None):
+        if U is None:
+            self._check(self.lib.m1_set_state(self.h, None))
+            return
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (4, *self.shape):
+            raise ValueError(f"state has shape {U.shape}, expected {(4, *self.shape)}")
+        self._check(self.lib.m1_set_state(self.h, _ptr(U)))
+
+    def get_state(self):
+        U = np.empty((4, *self.shape))
+        self._check(self.lib.m1_get_state(self.h, _ptr(U)))
+        return U
+
+    def get_dose(self):
+        d = np.empty(self.shape)
+        self._check(self.lib.m1_get_dose(self.h, _ptr(d)))
+        return d
+
+    # --- march ---------------------------------------------------------
+    def march(self, num: m1_numerics, snapshot_energies=()):
+        """(energies, {E: U}) after marching; dose and state stay on device."""
+        cap = 1 << 20
+        energies = np.empty(cap)
+        ne = C.c_int32(0)
+        snaps = np.ascontiguousarray([float(s) for s in snapshot_energies], dtype=np.float64)
+        ns = len(snaps)
+        snap_out = np.empty((ns, 4, *self.shape)) if ns else None
+        taken = np.zeros(max(ns, 1), dtype=np.int32)
+        self._check(self.lib.m1_march(self.h, C.byref(num), _ptr(energies), cap, C.byref(ne),
+                                      _ptr(snaps) if ns else None, ns, _ptr(snap_out),
+                                      _ptr(taken)))
+        snapshots = {}
+        for i in range(ns):
+            if taken[i]:
+                snapshots[float(snaps[i])] = snap_out[i]
+        return energies[: min(ne.value, cap)].copy(), snapshots
+
+    def spatial_operator(self, num, energy, boundary_state=None):
+        b = None
+        if boundary_state is not None:
+            b = np.ascontiguousarray(boundary_state, dtype=np.float64)
+            if b.shape != (4, self.shape[0], self.shape[1]):
+                raise ValueError(
+                    f"Boundary state has shape {b.shape}, expected {(4, *self.shape[:2])}."
+                )
+        rhs = np.empty((4, *self.shape))
+        self._check(self.lib.m1_spatial_operator(self.h, C.byref(num), float(energy), _ptr(b),
+                                                 _ptr(rhs)))
+        return rhs
+
+    def compute_energy_step(self, num, energy):
+        out = C.c_double()
+        self._check(self.lib.m1_compute_energy_step(self.h, C.byref(num), float(energy),
+                                                    C.byref(out)))
+        return out.value
+
+    def step(self, num, kind, energy, dE, want_edep=True):
+        edep = np.empty(self.shape) if want_edep else None
+        self._check(self.lib.m1_step(self.h, C.byref(num), int(kind), float(energy), float(dE),
+                                     _ptr(edep)))
+        return edep
+
+    # --- profiling -------------------------------------------------------
+    def set_profiling(self, on: bool):
+        self._check(self.lib.m1_set_profiling(self.h, int(bool(on))))
+
+    def reset_stats(self):
+        self._check(self.lib.m1_reset_stats(self.h))
+
+    def kernel_stats(self):
+        out = {}
+        for k in range(self.lib.m1_kernel_count()):
+            n = C.c_int64()
+            ms = C.c_double()
+            self._check(self.lib.m1_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
+            out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
+        return out

@@ -1,0 +1,80 @@
+"""Enumerations of the solver API (mirrors ``proton_solver/enums.py:4-51``).
+
+``Material`` carries two members the reference enum does not list, AIR and
+MUSCLE_SKELETAL: the reference ships their tables (``data/materials.csv:2,6``)
+but no enum entry.  All reference members keep their names and values.
+"""
+
+from enum import Enum, auto
+
+
+class Backend(Enum):
+    AUTO = auto()
+    CPU = auto()
+    GPU = auto()
+
+
+class Material(Enum):
+    WATER = "G4_WATER"
+    BONE_COMPACT = "G4_BONE_COMPACT_ICRU"
+    LUNG = "G4_LUNG_ICRP"
+    AIR = "G4_AIR"
+    MUSCLE_SKELETAL = "G4_MUSCLE_SKELETAL_ICRP"
+
+
+class SpatialScheme(Enum):
+    FIRST_ORDER = auto()
+    MUSCL = auto()
+    WENO3 = auto()
+    WENO_Z = auto()
+
+
+class EnergyScheme(Enum):
+    FORWARD_EULER = auto()
+    BACKWARD_EULER = auto()
+    TRAPEZOIDAL = auto()
+    SSPRK2 = auto()
+    SSPRK3 = auto()
+    IF_SSPRK3 = auto()
+
+
+class WaveSpeeds(Enum):
+    EXACT = auto()
+    UNIT = auto()
+
+
+class Reference(Enum):
+    NONE = auto()
+    ANALYTICAL = auto()
+    TOPAS = auto()
+    BFP_MC = auto()
+    SELF = auto()
+
+
+class CoefficientInterpolation(Enum):
+    """Interpolation of the tabulated scattering and removal coefficients."""
+
+    LINEAR = auto()
+    PCHIP = auto()
+
+
+class Scattering(Enum):
+    OFF = auto()
+    ROSSI = auto()
+    WENTZEL = auto()
+
+
+class Precision(Enum):
+    """Arithmetic of the device march (no reference counterpart).
+
+    FP64 reproduces the reference's all-double march with its operation
+    order, every operation separately rounded.  FP64_FAST is the same march
+    with fused multiply-adds and reciprocal constants (roundoff-level
+    differences).  FP32_BULK runs reconstruction, flux and stage algebra in
+    float and keeps the closure, the realizability projection, the
+    integrating factors and a compensated dose accumulation in double.
+    """
+
+    FP64 = auto()
+    FP32_BULK = auto()
+    FP64_FAST = auto()

@@ -1,0 +1,50 @@
+"""Shared helpers of the test suite (configs from fixtures, paths)."""
+
+import json
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "proton-dose-m1_amd"
+for p in (str(PKG), str(ROOT)):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+from proton_solver import (  # noqa: E402
+    Beam, Box, CoefficientInterpolation, EnergyScheme, Geometry, Material,
+    Numerics, Physics, Scattering, SolverConfig, SpatialScheme, WaveSpeeds,
+)
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def thesis_cases():
+    return json.loads((GOLDEN / "thesis_golden.json").read_text())["cases"]
+
+
+def config_from_fixture(c: dict) -> SolverConfig:
+    """SolverConfig of a thesis run recorded in thesis_golden.json."""
+    g = c["geometry"]
+    objects = [
+        Box(o["x_min"], o["x_max"], o["y_min"], o["y_max"], o["z_min"], o["z_max"],
+            Material[o["material"]])
+        for o in g["objects"]
+    ]
+    b = c["beam"]
+    n = c["numerics"]
+    p = c["physics"]
+    return SolverConfig(
+        geometry=Geometry(size=tuple(g["size"]), resolution=tuple(g["resolution"]),
+                          background_material=Material[g["background_material"]],
+                          objects=objects),
+        beam=Beam(n_protons=b["n_protons"], energy=b["energy"], energy_spread=b["energy_spread"],
+                  sigma=tuple(b["sigma"]), center=tuple(b["center"]),
+                  angular_spread=b["angular_spread"], cutoff_sigma=b["cutoff_sigma"]),
+        physics=Physics(scattering=Scattering[p["scattering"]], nuclear=p["nuclear"],
+                        coefficient_interpolation=CoefficientInterpolation[p["coefficient_interpolation"]]),
+        numerics=Numerics(spatial_scheme=SpatialScheme[n["spatial_scheme"]],
+                          energy_scheme=EnergyScheme[n["energy_scheme"]],
+                          wave_speeds=WaveSpeeds[n["wave_speeds"]], cfl=n["cfl"],
+                          energy_min=n["energy_min"], energy_window_sigma=n["energy_window_sigma"],
+                          max_energy_step=n["max_energy_step"]),
+    )
