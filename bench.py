@@ -1,0 +1,206 @@
+"""Benchmark of the MI355X M1 energy march (one JSON line on stdout).
+
+A step is one beamlet dose solve: a full energy march of the workload below,
+from E0 = E_beam + 5 sigma_E down to energy_min, with the problem (material
+ids, tables, inflow profile) resident in HBM before the timed region.
+
+Workload (the 150 MeV single-beamlet configuration of the metric): water,
+0.5 mm cells, 120 x 120 x 360 grid (6 x 6 x 18 cm), Gaussian beam sigma 4 mm,
+sigma_E 1 MeV, Wentzel-VI scattering + nuclear removal, IF-SSPRK3, WENO-Z,
+exact M1 speeds, cfl 0.9, max energy step 0.25 MeV, energy_min 1 MeV.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
+
+Reported: beamlet dose solves/s (value), cell-step updates/s, the HBM
+roofline of the dominant stage kernel (algorithmic bytes per launch over its
+HIP-event duration on the march stream) and the CPU oracle timed on a bounded
+sample of the same workload on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "proton-dose-m1_amd"))
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+MEV = 1.0
+
+# Algorithmic HBM bytes per cell of each FP64 stage kernel: every stage input
+# read once, every output written once, material id 1 byte:
+#   stage 0: read U (32) + ids (1), write V1 (32)
+#   stage 1: read V1 (32) + U (32) + ids (1), write V2 (32)
+#   stage 2: read V2 (32) + U (32) + V1.phi (8) + ids (1), write U (32), dose RMW (16)
+STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
+
+
+def workload_config(precision):
+    from proton_solver import (
+        Beam, DeviceOptions, EnergyScheme, Geometry, Numerics, Physics, Scattering,
+        SolverConfig, SpatialScheme, WaveSpeeds,
+    )
+
+    return SolverConfig(
+        geometry=Geometry(size=(0.06, 0.06, 0.18), resolution=(120, 120, 360)),
+        beam=Beam(n_protons=1e6, energy=150.0, energy_spread=1.0, sigma=(0.004, 0.004),
+                  center=(0.0, 0.0), angular_spread=0.0, cutoff_sigma=5.0),
+        physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
+        numerics=Numerics(spatial_scheme=SpatialScheme.WENO_Z,
+                          energy_scheme=EnergyScheme.IF_SSPRK3, wave_speeds=WaveSpeeds.EXACT,
+                          max_energy_step=0.25, cfl=0.9, energy_min=1.0),
+        device=DeviceOptions(precision=precision),
+    )
+
+
+def host_problem(cfg):
+    from proton_solver.numerics.boundary import BeamInflow
+    from proton_solver.numerics.grid import Grid
+    from proton_solver.physics.materials import MaterialMap
+
+    grid = Grid.from_geometry(cfg.geometry, np)
+    mm = MaterialMap.from_geometry(cfg.geometry, grid, np)
+    inflow = BeamInflow.from_beam(cfg.beam, grid, np)
+    return grid, mm, inflow
+
+
+def cpu_baseline(cfg, n_steps_full, sample_steps):
+    """Oracle (C/OpenMP FP64 restatement) on the first energy steps of the march."""
+    from oracle.oracle import OracleProblem
+
+    grid, mm, inflow = host_problem(cfg)
+    op = OracleProblem(grid, mm, inflow, cfg)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    op.march(max_steps=1)  # warm-up (page-in, thread pool)
+    t0 = time.perf_counter()
+    _, _, E, _ = op.march(max_steps=sample_steps)
+    dt = time.perf_counter() - t0
+    steps = len(E) - 1
+    n = int(np.prod(grid.shape))
+    per_step = dt / steps
+    return {
+        "value": 1.0 / (per_step * n_steps_full),
+        "unit": "beamlet dose solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"first {steps} of {n_steps_full} energy steps of the same "
+                   f"{grid.shape[0]}x{grid.shape[1]}x{grid.shape[2]} march "
+                   f"({dt:.1f} s), extrapolated to the full march"),
+        "cell_steps_per_s": n * steps / dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64_fast"])
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X: the march of one beamlet runs on one GPU and "
+              "this build has no multi-GPU path (beamlet sharding is described in DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from proton_solver import Precision
+    from proton_solver import _native
+
+    precision = {"fp64": Precision.FP64, "fp64_fast": Precision.FP64_FAST}[args.precision]
+    cfg = workload_config(precision)
+    grid, mm, inflow = host_problem(cfg)
+    n_cells = int(np.prod(grid.shape))
+
+    ctx = _native.Context(0)
+    ctx.set_problem(grid, mm, inflow, cfg.beam)
+    num = _native.numerics_struct(cfg)
+
+    for _ in range(args.warmup):
+        ctx.set_state(None)
+        ctx.march(num)
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    steps_per_march = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.set_state(None)
+        E, _ = ctx.march(num)
+        steps_per_march.append(len(E) - 1)
+    elapsed = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    dose = ctx.get_dose()
+
+    n_energy_steps = steps_per_march[-1]
+    ms_per_step = 1000.0 * elapsed / args.steps
+    solves_per_s = args.steps / elapsed
+    cell_steps = n_cells * sum(steps_per_march)
+
+    # Dominant stage kernel and its HBM roofline.
+    best = None
+    for name, per_cell in STAGE_BYTES_FP64.items():
+        launches, total_ms = stats.get(name, (0, 0.0))
+        if launches == 0:
+            continue
+        avg_s = total_ms / launches / 1000.0
+        achieved = per_cell * n_cells / avg_s / 1e9
+        if best is None or total_ms > best[1]:
+            best = (name, total_ms, achieved, avg_s, per_cell)
+    roofline = None
+    if best is not None:
+        name, total_ms, achieved, avg_s, per_cell = best
+        roofline = {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+            "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
+            "algorithmic_bytes_per_cell": per_cell,
+        }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfg, n_energy_steps, args.cpu_sample_steps)
+
+    line = {
+        "metric": "beamlet dose solves/s at <=0.5% max-dose err vs plain FP64; cell-step updates/s",
+        "value": solves_per_s,
+        "unit": "beamlet dose solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic water phantom (no external data)",
+        "config": {
+            "workload": "150 MeV water pencil beam, 0.5 mm, 120x120x360, Wentzel+nuclear, "
+                        "IF-SSPRK3, one beamlet per step",
+            "precision": args.precision,
+            "grid": list(grid.shape),
+            "energy_steps_per_solve": n_energy_steps,
+            "max_dose_Gy": float(dose.max()),
+        },
+        "cell_steps_per_s": cell_steps / elapsed,
+        "kernel_ms_per_solve": {k: round(v[1] / max(args.steps, 1), 3) for k, v in stats.items()
+                                if v[0]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -726,6 +726,16 @@ static int conserved_inflow(const or_problem *p, double energy, double *B)
     return 0;
 }
 
+void or_rk3_trace_combination(long n, const double *B0, const double *B13,
+                              const double *B23, const double *B1,
+                              double *Bs1, double *Bs2)
+{
+    for (long i = 0; i < n; ++i) {
+        Bs1[i] = 0.5 * (-9.0 * B0[i] + 18.0 * B13[i] - 9.0 * B23[i] + 2.0 * B1[i]);
+        Bs2[i] = 0.25 * (11.0 * B0[i] - 27.0 * B13[i] + 27.0 * B23[i] - 7.0 * B1[i]);
+    }
+}
+
 /* energy.py:107-230 and 348-455.  When ts/tn arrays are NULL the plain
  * SSPRK3 traces are produced; otherwise the IF-SSPRK3 traces with depths
  * for c = 1/3, 1/2, 2/3, 1 in that order (each n_mat long). */
@@ -761,10 +771,7 @@ static int stage_inflow(const or_problem *p, double energy, double dE,
             }
         }
     }
-    for (long i = 0; i < 4 * m; ++i) {
-        Bs1[i] = 0.5 * (-9.0 * B0[i] + 18.0 * B13[i] - 9.0 * B23[i] + 2.0 * B1[i]);
-        Bs2[i] = 0.25 * (11.0 * B0[i] - 27.0 * B13[i] + 27.0 * B23[i] - 7.0 * B1[i]);
-    }
+    or_rk3_trace_combination(4 * m, B0, B13, B23, B1, Bs1, Bs2);
     if (ts4) {
         double f1p[64], f1j[64], f2p[64], f2j[64];
         source_factors(nm, ts4[3], tn4[3], 0, f1p, f1j);

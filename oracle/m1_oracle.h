@@ -105,6 +105,11 @@ void or_reconstruct(const double *U_ext, int nx, int ny, int nz, int axis,
 int or_spatial_operator(const or_problem *p, int wave_mode, const double *U,
                         double energy, const double *boundary_state,
                         double *rhs);
+/* energy.py:180-202 / 412-424: stage-1 and stage-2 boundary values from
+ * four samples of the (transformed) trace at tau/dE = 0, 1/3, 2/3, 1. */
+void or_rk3_trace_combination(long n, const double *B0, const double *B13,
+                              const double *B23, const double *B1,
+                              double *Bs1, double *Bs2);
 /* boundary.py:82-114 */
 int or_inflow_state(const or_problem *p, double energy, double *U_in);
 /* energy.py:539-660 (vacuum_relative_tolerance=None). */

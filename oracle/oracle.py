@@ -96,6 +96,7 @@ def lib():
         L.or_energy_march.argtypes = [P, Cf, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int),
                                       _dp, C.c_int, _dp, _ip, C.c_int]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_rk3_trace_combination.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -301,6 +302,38 @@ def reconstruct(U_ext, axis):
     R = np.empty(shape)
     lib().or_reconstruct(_ptr(U_ext), nx, ny, nz, axis, _ptr(L), _ptr(R))
     return L, R
+
+
+def rk3_trace_combination(B0, B13, B23, B1):
+    arrs = [np.ascontiguousarray(b, dtype=np.float64) for b in (B0, B13, B23, B1)]
+    o1 = np.empty_like(arrs[0])
+    o2 = np.empty_like(arrs[0])
+    lib().or_rk3_trace_combination(arrs[0].size, *[_ptr(a) for a in arrs], _ptr(o1), _ptr(o2))
+    return o1, o2
+
+
+def material_from_arrays(sp, tr, nu, density, radiation_length):
+    """or_material from (E, v) pairs; slopes computed by the oracle."""
+    keep = []
+    tabs = []
+    for E, v in (sp, tr, nu):
+        t, k = make_table(E, v, True)
+        tabs.append(t)
+        keep += k
+    return or_material(*tabs, float(density), float(radiation_length)), keep
+
+
+def scattering_depth(mat, model, energy, dE, interp="LINEAR"):
+    out = C.c_double()
+    _check(lib().or_scattering_depth(C.byref(mat), _SCAT[model], _INTERP[interp], energy, dE,
+                                     C.byref(out)))
+    return out.value
+
+
+def nuclear_depth(mat, energy, dE, interp="LINEAR"):
+    out = C.c_double()
+    _check(lib().or_nuclear_depth(C.byref(mat), _INTERP[interp], energy, dE, C.byref(out)))
+    return out.value
 
 
 def table_eval(E, v, energy, interp="PCHIP"):
