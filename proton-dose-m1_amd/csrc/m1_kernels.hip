@@ -52,15 +52,40 @@ constexpr double EIG_EPS = 1.0e-10;   // wave_speeds.py:10
 constexpr double WENO_EPS = 1.0e-12;  // weno_z.py:8
 constexpr double REAL_EPS = 1.0e-10;  // weno_z.py:10, realizability.py:182
 
+// ------------------------------------------------------- exact helpers
+// x / D for the constant divisors of the reconstruction and closure.  The
+// quotient x * RN(1/D) corrected once with its exact FMA residual is the
+// correctly rounded x / D for every normal x (Markstein's division by a
+// constant with a correctly rounded reciprocal; checked on 8e8 random
+// operands for D = 3, 4, 6, 12).  The residual can underflow for tiny x, and
+// the sign of a zero quotient must follow x, so |x| < 2^-960, zeros and NaNs
+// take the IEEE division.  The fast build multiplies by the reciprocal.
+template <int D>
+__device__ __forceinline__ double div_by(double x)
+{
+    constexpr double d = D;
+    constexpr double r = 1.0 / D;
+#if M1_EXACT
+    double q = x * r;
+    const double t = __builtin_fma(-q, d, x);
+    q = __builtin_fma(t, r, q);
+    if (__builtin_expect(!(fabs(x) >= 0x1p-960), 0))
+        q = x / d;
+    return q;
+#else
+    return x * r;
+#endif
+}
+
 // ------------------------------------------------------------------ WENO-Z
 // weno_z.py:13-87: left state at the face between v2 and v3.
 __device__ __forceinline__ double weno_left(double v0, double v1, double v2, double v3,
                                             double v4)
 {
 #if M1_EXACT
-    const double p0 = v0 / 3.0 - 7.0 * v1 / 6.0 + 11.0 * v2 / 6.0;
-    const double p1 = -v1 / 6.0 + 5.0 * v2 / 6.0 + v3 / 3.0;
-    const double p2 = v2 / 3.0 + 5.0 * v3 / 6.0 - v4 / 6.0;
+    const double p0 = div_by<3>(v0) - div_by<6>(7.0 * v1) + div_by<6>(11.0 * v2);
+    const double p1 = div_by<6>(-v1) + div_by<6>(5.0 * v2) + div_by<3>(v3);
+    const double p2 = div_by<3>(v2) + div_by<6>(5.0 * v3) - div_by<6>(v4);
 #else
     const double p0 = v0 * (1.0 / 3.0) - v1 * (7.0 / 6.0) + v2 * (11.0 / 6.0);
     const double p1 = v2 * (5.0 / 6.0) - v1 * (1.0 / 6.0) + v3 * (1.0 / 3.0);
@@ -108,7 +133,61 @@ __device__ __forceinline__ void project(double& p, double& x, double& y, double&
 }
 
 // ------------------------------------------------------------ closure/HLL
-// wave_speeds.py:37-103 / flux.py:63-135 for one state along `axis`.
+// Levermore closure of one state (flux.py:10-61).  q = |J| / phi is kept so
+// that the eigenvalues can reuse f, chi and n when their own clamps
+// (phi >= EPS, f <= 1 - EPS) leave the same values.
+struct Closure {
+    double jn, q, chi, nx, ny, nz, iso, dir;
+};
+
+__device__ __forceinline__ double levermore_chi(double f)
+{
+    return div_by<3>(5.0 - 2.0 * sqrt(4.0 - 3.0 * f * f));
+}
+
+__device__ __forceinline__ Closure closure(double phi, double jx, double jy, double jz)
+{
+    Closure c;
+    c.jn = sqrt(jx * jx + jy * jy + jz * jz);
+    double f;
+    if (phi > 0.0) {
+        c.q = c.jn / phi;
+        f = c.q;
+        if (f < 0.0)
+            f = 0.0;
+        else if (f > 1.0)
+            f = 1.0;
+    } else {
+        c.q = 0.0;
+        f = 0.0;
+    }
+    c.chi = levermore_chi(f);
+    if (c.jn > 0.0) {
+        c.nx = jx / c.jn;
+        c.ny = jy / c.jn;
+        c.nz = jz / c.jn;
+    } else {
+        c.nx = c.ny = c.nz = 0.0;
+    }
+    c.iso = 0.5 * (1.0 - c.chi);
+    c.dir = 0.5 * (3.0 * c.chi - 1.0);
+    return c;
+}
+
+// Extreme M1 eigenvalues along an axis from (f, mu, chi) (wave_speeds.py:72-103).
+__device__ __forceinline__ void eigen_core(double f, double mu, double chi, double& lmin,
+                                           double& lmax)
+{
+    const double den = 5.0 - 3.0 * chi;
+    double disc = 4.0 * f * f * mu * mu - den * (mu * mu * (3.0 * chi - 1.0) - 2.0 * (1.0 - chi));
+    if (disc < 0.0)
+        disc = 0.0;
+    const double root = sqrt(disc);
+    lmin = (2.0 * f * mu - root) / den;
+    lmax = (2.0 * f * mu + root) / den;
+}
+
+// wave_speeds.py:37-103 / flux.py:63-135 for one state along an axis.
 __device__ __forceinline__ void eigen(double phi, double jnormal, double jn, double& lmin,
                                       double& lmax)
 {
@@ -125,47 +204,22 @@ __device__ __forceinline__ void eigen(double phi, double jnormal, double jn, dou
         f = 0.0;
     else if (f > 1.0 - EIG_EPS)
         f = 1.0 - EIG_EPS;
-    const double mu = jnormal / jn;
-    const double chi = (5.0 - 2.0 * sqrt(4.0 - 3.0 * f * f)) / 3.0;
-    const double den = 5.0 - 3.0 * chi;
-    double disc = 4.0 * f * f * mu * mu - den * (mu * mu * (3.0 * chi - 1.0) - 2.0 * (1.0 - chi));
-    if (disc < 0.0)
-        disc = 0.0;
-    const double root = sqrt(disc);
-    lmin = (2.0 * f * mu - root) / den;
-    lmax = (2.0 * f * mu + root) / den;
+    eigen_core(f, jnormal / jn, levermore_chi(f), lmin, lmax);
 }
 
-struct Closure {
-    double jn, nx, ny, nz, iso, dir;
-};
-
-// flux.py:10-61
-__device__ __forceinline__ Closure closure(double phi, double jx, double jy, double jz)
+// Same eigenvalues reusing the closure of the state: with phi >= EPS and
+// |J| / phi <= 1 - EPS the eigenvalue clamps are inactive and f, chi and
+// mu = J_axis / |J| are the closure's values bit for bit.
+template <int AXIS>
+__device__ __forceinline__ void eigen_closure(double phi, const Closure& c, double jnormal,
+                                              double& lmin, double& lmax)
 {
-    Closure c;
-    c.jn = sqrt(jx * jx + jy * jy + jz * jz);
-    double f;
-    if (phi > 0.0) {
-        f = c.jn / phi;
-        if (f < 0.0)
-            f = 0.0;
-        else if (f > 1.0)
-            f = 1.0;
+    if (phi >= EIG_EPS && c.jn >= EIG_EPS && c.q <= 1.0 - EIG_EPS) {
+        const double mu = AXIS == 0 ? c.nx : (AXIS == 1 ? c.ny : c.nz);
+        eigen_core(c.q, mu, c.chi, lmin, lmax);
     } else {
-        f = 0.0;
+        eigen(phi, jnormal, c.jn, lmin, lmax);
     }
-    const double chi = (5.0 - 2.0 * sqrt(4.0 - 3.0 * f * f)) / 3.0;
-    if (c.jn > 0.0) {
-        c.nx = jx / c.jn;
-        c.ny = jy / c.jn;
-        c.nz = jz / c.jn;
-    } else {
-        c.nx = c.ny = c.nz = 0.0;
-    }
-    c.iso = 0.5 * (1.0 - chi);
-    c.dir = 0.5 * (3.0 * chi - 1.0);
-    return c;
 }
 
 // Physical flux along AXIS (flux.py:199-309).
@@ -204,8 +258,8 @@ __device__ __forceinline__ void hll(const double L[4], const double R[4], int wa
     double sl, sr;
     if (wave_mode == 0) {
         double lminl, lmaxl, lminr, lmaxr;
-        eigen(L[0], L[AXIS + 1], cl.jn, lminl, lmaxl);
-        eigen(R[0], R[AXIS + 1], cr.jn, lminr, lmaxr);
+        eigen_closure<AXIS>(L[0], cl, L[AXIS + 1], lminl, lmaxl);
+        eigen_closure<AXIS>(R[0], cr, R[AXIS + 1], lminr, lmaxr);
         sl = lminl < lminr ? lminl : lminr;
         sr = lmaxl > lmaxr ? lmaxl : lmaxr;
         if (sl > 0.0)
@@ -586,13 +640,24 @@ __global__ void __launch_bounds__(CFL_THREADS) cfl_kernel(CflArgs a)
         } else {
             const double phi = a.U[i], jx = a.U[i + N], jy = a.U[i + 2 * N], jz = a.U[i + 3 * N];
             const double jn = sqrt(jx * jx + jy * jy + jz * jz);
-            double lo, hi;
-            eigen(phi, jx, jn, lo, hi);
-            const double ax = fmax(fabs(lo), fabs(hi));
-            eigen(phi, jy, jn, lo, hi);
-            const double ay = fmax(fabs(lo), fabs(hi));
-            eigen(phi, jz, jn, lo, hi);
-            double az = fmax(fabs(lo), fabs(hi));
+            double ax, ay, az, lo, hi;
+            if (jn < EIG_EPS) {
+                ax = ay = az = 1.0 / sqrt(3.0);
+            } else {
+                // f and chi are the same for the three axes.
+                double f = jn / (phi < EIG_EPS ? EIG_EPS : phi);
+                if (f < 0.0)
+                    f = 0.0;
+                else if (f > 1.0 - EIG_EPS)
+                    f = 1.0 - EIG_EPS;
+                const double chi = levermore_chi(f);
+                eigen_core(f, jx / jn, chi, lo, hi);
+                ax = fmax(fabs(lo), fabs(hi));
+                eigen_core(f, jy / jn, chi, lo, hi);
+                ay = fmax(fabs(lo), fabs(hi));
+                eigen_core(f, jz / jn, chi, lo, hi);
+                az = fmax(fabs(lo), fabs(hi));
+            }
             if (i % nz == 0) {
                 const double pin = a.spatial[i / nz] * ed;
                 const double jin = rf * pin;
