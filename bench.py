@@ -9,7 +9,12 @@ Workload (the 150 MeV single-beamlet configuration of the metric): water,
 sigma_E 1 MeV, Wentzel-VI scattering + nuclear removal, IF-SSPRK3, WENO-Z,
 exact M1 speeds, cfl 0.9, max energy step 0.25 MeV, energy_min 1 MeV.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32_bulk|fp64|fp64_fast]
+
+The measured march is FP32-bulk by default (float reconstruction, flux and
+stage algebra; double closure, projection and dose accumulation); one plain
+FP64 solve of the same beamlet, outside the timed region, states its
+max-dose error and gamma(1%/1 mm) pass rate and the speed-up over FP64.
 
 Reported: beamlet dose solves/s (value), cell-step updates/s, the HBM
 roofline of the dominant stage kernel (algorithmic bytes per launch over its
@@ -41,6 +46,11 @@ MEV = 1.0
 #   stage 1: read V1 (32) + U (32) + ids (1), write V2 (32)
 #   stage 2: read V2 (32) + U (32) + V1.phi (8) + ids (1), write U (32), dose RMW (16)
 STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
+# FP32-bulk: float state (16 B per cell-state), dose as a compensated float pair:
+#   stage 0: read U (16) + ids (1), write V1 (16)
+#   stage 1: read V1 (16) + U (16) + ids (1), write V2 (16)
+#   stage 2: read V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16), dose pair RMW (16)
+STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 69}
 
 
 def workload_config(precision):
@@ -103,7 +113,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64_fast"])
+    ap.add_argument("--precision", default="fp32_bulk", choices=["fp32_bulk", "fp64", "fp64_fast"])
+    ap.add_argument("--no-fp64-reference", action="store_true",
+                    help="skip the FP64 solve used to state the error of the measured march")
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -118,7 +130,8 @@ def main():
     from proton_solver import Precision
     from proton_solver import _native
 
-    precision = {"fp64": Precision.FP64, "fp64_fast": Precision.FP64_FAST}[args.precision]
+    precision = {"fp64": Precision.FP64, "fp64_fast": Precision.FP64_FAST,
+                 "fp32_bulk": Precision.FP32_BULK}[args.precision]
     cfg = workload_config(precision)
     grid, mm, inflow = host_problem(cfg)
     n_cells = int(np.prod(grid.shape))
@@ -126,6 +139,18 @@ def main():
     ctx = _native.Context(0)
     ctx.set_problem(grid, mm, inflow, cfg.beam)
     num = _native.numerics_struct(cfg)
+    stage_bytes = STAGE_BYTES_FP32 if precision is Precision.FP32_BULK else STAGE_BYTES_FP64
+
+    # Plain FP64 march of the same beamlet: the accuracy reference of the
+    # measured march and the denominator of its speed-up (outside the timed region).
+    fp64 = None
+    if not args.no_fp64_reference and precision is not Precision.FP64:
+        num64 = _native.numerics_struct(cfg, _native.FP64)
+        ctx.set_state(None)
+        t0 = time.perf_counter()
+        E64, _ = ctx.march(num64)
+        fp64 = {"seconds": time.perf_counter() - t0, "steps": len(E64) - 1,
+                "dose": ctx.get_dose()}
 
     for _ in range(args.warmup):
         ctx.set_state(None)
@@ -150,7 +175,7 @@ def main():
 
     # Dominant stage kernel and its HBM roofline.
     best = None
-    for name, per_cell in STAGE_BYTES_FP64.items():
+    for name, per_cell in stage_bytes.items():
         launches, total_ms = stats.get(name, (0, 0.0))
         if launches == 0:
             continue
@@ -168,6 +193,22 @@ def main():
             "algorithmic_bytes_per_cell": per_cell,
         }
 
+    accuracy = None
+    if fp64 is not None:
+        from proton_solver import metrics
+
+        ref = fp64["dose"]
+        accuracy = {
+            "reference": "plain FP64 march on the same GPU (bit-identical to the CPU oracle)",
+            "max_dose_rel_err": metrics.maximum_dose_relative_difference(dose, ref),
+            "max_abs_diff_over_max": metrics.max_abs_difference_over_max(dose, ref),
+            "gamma_1pct_1mm_pass_percent": metrics.gamma_pass_rate(
+                dose, ref, (grid.dx, grid.dy, grid.dz), 1.0, 1.0, 10.0),
+            "fp64_solve_s": fp64["seconds"],
+            "fp64_energy_steps": fp64["steps"],
+            "speedup_vs_fp64_march": fp64["seconds"] / (elapsed / args.steps),
+        }
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, n_energy_steps, args.cpu_sample_steps)
@@ -183,7 +224,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp64_fast": "f64",
+                  "fp32_bulk": "f32 bulk / f64 closure+projection+dose"}[args.precision],
         "data": "synthetic water phantom (no external data)",
         "config": {
             "workload": "150 MeV water pencil beam, 0.5 mm, 120x120x360, Wentzel+nuclear, "
@@ -196,6 +238,7 @@ def main():
         "cell_steps_per_s": cell_steps / elapsed,
         "kernel_ms_per_solve": {k: round(v[1] / max(args.steps, 1), 3) for k, v in stats.items()
                                 if v[0]},
+        "accuracy_vs_fp64": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

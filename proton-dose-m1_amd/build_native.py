@@ -42,6 +42,7 @@ def _units():
          ["-DM1_EXACT=1", "-ffp-contract=off"]),
         (CSRC / "m1_kernels.hip", BUILD / "m1_kernels_fast.o",
          ["-DM1_EXACT=0", "-ffp-contract=fast"]),
+        (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=fast"]),
         (CSRC / "m1_march.hip", BUILD / "m1_march.o", ["-ffp-contract=off"]),
     ]
 

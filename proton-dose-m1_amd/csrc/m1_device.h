@@ -38,6 +38,26 @@ struct StepCoef {
     double i23p[MAXMAT], i23j[MAXMAT];  // C(tau(2dE/3))^-1
     double rho[MAXMAT];                 // density, kg/m^3
     double hn[MAXMAT], hs[MAXMAT];      // Strang half-step factors (energy.py:1168-1191)
+    // Normalisation of the FP32-bulk march: the device state is U / alpha.
+    double alpha;            // peak inflow fluence of the march
+    double inv_alpha;
+    double eig_eps;          // 1e-10 / alpha (wave_speeds.py:10 in scaled units)
+    double weno_eps;         // 1e-12 / alpha^2 (weno_z.py:8 in scaled units)
+};
+
+// Float copy of the per-cell stage scalars for the FP32-bulk kernels.
+struct StepCoef32 {
+    float dE;
+    float weno_eps;
+    float S0[MAXMAT];
+    float g1p[MAXMAT], g1j[MAXMAT];     // C(tau(dE)) / S(E1): V1 -> U1
+    float g2p[MAXMAT], g2j[MAXMAT];     // C(tau(dE/2)) / S(E2): V2 -> U2
+    float i1p[MAXMAT], i1j[MAXMAT];
+    float i2p[MAXMAT], i2j[MAXMAT];
+    float n1p[MAXMAT], n1j[MAXMAT];     // C(tau(dE)) / S(E1): V_new -> U_new
+    float S1[MAXMAT], S2[MAXMAT];
+    float dose_scale[MAXMAT];           // alpha * MEV / rho
+    double eig_eps;
 };
 
 struct GridDims {
@@ -93,6 +113,55 @@ struct ScaleArgs {
     const StepCoef* coef;
     int nuclear, scattering;
 };
+
+struct StageArgs32 {
+    GridDims g;
+    const float* src;       // stencil input: U (STAGE0/RHS/FE), V1 (STAGE1), V2 (STAGE2)
+    float* U;               // accepted state (scaled); STAGE2 writes U^{n+1}
+    const float* V1;
+    float* out;             // STAGE0: V1, STAGE1: V2, RHS: rhs, FE: U^{n+1}
+    float* dose_hi;         // compensated dose pair, Gy (nullable)
+    float* dose_lo;
+    float* edep;            // scaled energy deposition (nullable)
+    const uint8_t* ids;
+    const float* bnd;       // (4, nx, ny) scaled boundary state of this stage
+    const StepCoef32* coef;
+    int wave_mode;
+};
+
+struct CflArgs32 {
+    GridDims g;
+    const float* U;
+    const uint8_t* ids;
+    const double* spatial;
+    const StepCoef* coef;
+    unsigned long long* out;
+    int wave_mode;
+};
+
+struct BoundaryArgs32 {
+    GridDims g;
+    const double* spatial;
+    const uint8_t* ids;
+    const StepCoef* coef;
+    float* bnd;
+    int integrating_factor;
+};
+
+namespace f32 {
+hipError_t launch_stage(int kind, bool integrating_factor, const StageArgs32& a, hipStream_t s);
+hipError_t launch_cfl(const CflArgs32& a, hipStream_t s);
+hipError_t launch_boundary(const BoundaryArgs32& a, hipStream_t s);
+// dose64 = double(hi) + double(lo)
+hipError_t launch_finalize_dose(const float* hi, const float* lo, double* dose, long n,
+                                hipStream_t s);
+// state64 = alpha * double(U32)
+hipError_t launch_widen_state(const float* U, double alpha, double* out, long n, hipStream_t s);
+hipError_t launch_narrow_state(const double* U, double inv_alpha, float* out, long n,
+                               hipStream_t s);
+hipError_t launch_scale(long n, float* U, const uint8_t* ids, const StepCoef* coef,
+                        int nuclear, int scattering, hipStream_t s);
+}  // namespace f32
 
 #define M1_DECLARE_LAUNCHERS(NS)                                                          \
     namespace NS {                                                                        \

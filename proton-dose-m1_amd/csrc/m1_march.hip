@@ -69,6 +69,12 @@ struct m1_ctx {
     double *U = nullptr, *V1 = nullptr, *V2 = nullptr, *work = nullptr, *dose = nullptr,
            *edep = nullptr, *bnd = nullptr, *spatial = nullptr;
     uint8_t* ids = nullptr;
+    // FP32-bulk buffers, allocated on first use.
+    float *U32 = nullptr, *V1_32 = nullptr, *V2_32 = nullptr, *work32 = nullptr,
+          *dose_hi = nullptr, *dose_lo = nullptr, *edep32 = nullptr, *bnd32 = nullptr;
+    double alpha = 1.0;
+    m1::StepCoef32* d_step32 = nullptr;
+    m1::StepCoef32* h_step32 = nullptr;
     StepCoef* d_step = nullptr;
     StepCoef* d_cfl = nullptr;
     unsigned long long* d_red = nullptr;
@@ -93,10 +99,13 @@ struct m1_ctx {
 void m1_ctx::free_problem()
 {
     for (void* p : {(void*)U, (void*)V1, (void*)V2, (void*)work, (void*)dose, (void*)edep,
-                    (void*)bnd, (void*)spatial, (void*)ids})
+                    (void*)bnd, (void*)spatial, (void*)ids, (void*)U32, (void*)V1_32,
+                    (void*)V2_32, (void*)work32, (void*)dose_hi, (void*)dose_lo, (void*)edep32,
+                    (void*)bnd32})
         if (p)
             (void)hipFree(p);
     U = V1 = V2 = work = dose = edep = bnd = spatial = nullptr;
+    U32 = V1_32 = V2_32 = work32 = dose_hi = dose_lo = edep32 = bnd32 = nullptr;
     ids = nullptr;
     have_problem = false;
 }
@@ -107,10 +116,10 @@ m1_ctx::~m1_ctx()
     if (stream)
         (void)hipStreamSynchronize(stream);
     free_problem();
-    for (void* p : {(void*)d_step, (void*)d_cfl, (void*)d_red})
+    for (void* p : {(void*)d_step, (void*)d_cfl, (void*)d_red, (void*)d_step32})
         if (p)
             (void)hipFree(p);
-    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red})
+    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red, (void*)h_step32})
         if (p)
             (void)hipHostFree(p);
     for (auto& e : pool)
@@ -202,9 +211,7 @@ Num read_numerics(const m1_numerics* n)
         throw ValueError{"Unknown scattering model."};
     if (r.interp != M1_INTERP_LINEAR && r.interp != M1_INTERP_PCHIP)
         throw ValueError{"Unknown coefficient interpolation."};
-    if (r.precision == M1_FP32_BULK)
-        throw NotImplemented{"FP32_BULK precision is not available in this build."};
-    if (r.precision != M1_FP64 && r.precision != M1_FP64_FAST)
+    if (r.precision != M1_FP64 && r.precision != M1_FP64_FAST && r.precision != M1_FP32_BULK)
         throw ValueError{"Unknown precision."};
     if (!(std::isfinite(r.max_step) && std::isfinite(r.cfl) && std::isfinite(r.emin) &&
           std::isfinite(r.window)))
@@ -541,6 +548,241 @@ void dev_alloc(T** p, size_t count, const char* what)
     hip_check(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)), what);
 }
 
+// ------------------------------------------------------------- FP32 bulk
+// The FP32-bulk march keeps the device state as U / alpha in float, with
+// alpha the peak inflow fluence max(profile) * N(E_beam) (so phi <= ~1).
+
+void ensure_f32(m1_ctx* c)
+{
+    if (c->U32)
+        return;
+    const size_t N = size_t(c->N), m2 = size_t(c->nx) * c->ny;
+    dev_alloc(&c->U32, 4 * N, "hipMalloc state (f32)");
+    dev_alloc(&c->V1_32, 4 * N, "hipMalloc stage 1 (f32)");
+    dev_alloc(&c->V2_32, 4 * N, "hipMalloc stage 2 (f32)");
+    dev_alloc(&c->work32, 4 * N, "hipMalloc work (f32)");
+    dev_alloc(&c->dose_hi, N, "hipMalloc dose (f32)");
+    dev_alloc(&c->dose_lo, N, "hipMalloc dose (f32)");
+    dev_alloc(&c->edep32, N, "hipMalloc deposition (f32)");
+    dev_alloc(&c->bnd32, 12 * m2, "hipMalloc boundary (f32)");
+    if (!c->d_step32) {
+        dev_alloc(&c->d_step32, 1, "hipMalloc coefficients (f32)");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_step32), sizeof(m1::StepCoef32)),
+                  "hipHostMalloc");
+    }
+}
+
+double march_alpha(const m1_ctx* c, const std::vector<double>& profile_max)
+{
+    const double pk = profile_max.empty() ? 0.0 : profile_max[0];
+    const double ed = m1::energy_density(c->beam_energy, c->beam_energy, c->energy_spread);
+    const double a = pk * ed;
+    return a > 0.0 && std::isfinite(a) ? a : 1.0;
+}
+
+void set_norm(const m1_ctx* c, StepCoef& s)
+{
+    s.alpha = c->alpha;
+    s.inv_alpha = 1.0 / c->alpha;
+    s.eig_eps = 1.0e-10 / c->alpha;
+    s.weno_eps = 1.0e-12 / (c->alpha * c->alpha);
+}
+
+// Float stage scalars from the double ones.
+void fill_step32(const m1_ctx* c, const StepCoef& d, m1::StepCoef32& f)
+{
+    std::memset(&f, 0, sizeof f);
+    f.dE = float(d.dE);
+    f.weno_eps = float(std::max(d.weno_eps, 1.0e-37));
+    f.eig_eps = d.eig_eps;
+    for (size_t m = 0; m < c->mats.size(); ++m) {
+        f.S0[m] = float(d.S0[m]);
+        f.S1[m] = float(d.S1[m]);
+        f.S2[m] = float(d.S2[m]);
+        f.g1p[m] = float(d.f1p[m] / d.S1[m]);
+        f.g1j[m] = float(d.f1j[m] / d.S1[m]);
+        f.g2p[m] = float(d.f2p[m] / d.S2[m]);
+        f.g2j[m] = float(d.f2j[m] / d.S2[m]);
+        f.i1p[m] = float(d.i1p[m]);
+        f.i1j[m] = float(d.i1j[m]);
+        f.i2p[m] = float(d.i2p[m]);
+        f.i2j[m] = float(d.i2j[m]);
+        f.n1p[m] = f.g1p[m];
+        f.n1j[m] = f.g1j[m];
+        f.dose_scale[m] = float(c->alpha * m1::MEV_TO_JOULE / d.rho[m]);
+    }
+}
+
+void upload_step32(m1_ctx* c)
+{
+    set_norm(c, *c->h_step);
+    fill_step32(c, *c->h_step, *c->h_step32);
+    hip_check(hipMemcpyAsync(c->d_step, c->h_step, sizeof(StepCoef), hipMemcpyHostToDevice,
+                             c->stream),
+              "coefficient upload");
+    hip_check(hipMemcpyAsync(c->d_step32, c->h_step32, sizeof(m1::StepCoef32),
+                             hipMemcpyHostToDevice, c->stream),
+              "coefficient upload (f32)");
+}
+
+m1::StageArgs32 base_args32(const m1_ctx* c, const Num& n)
+{
+    m1::StageArgs32 a{};
+    a.g = dims(c);
+    a.ids = c->ids;
+    a.coef = c->d_step32;
+    a.wave_mode = n.wave;
+    return a;
+}
+
+void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
+{
+    {
+        m1::BoundaryArgs32 b{dims(c), c->spatial, c->ids, c->d_step, c->bnd32, ifac ? 1 : 0};
+        Timed t(c, K_BOUNDARY);
+        hip_check(m1::f32::launch_boundary(b, c->stream), "boundary kernel (f32)");
+    }
+    const long m2 = long(c->nx) * c->ny;
+    m1::StageArgs32 a = base_args32(c, n);
+    a.U = c->U32;
+    a.V1 = c->V1_32;
+    {
+        a.src = c->U32;
+        a.out = c->V1_32;
+        a.bnd = c->bnd32;
+        Timed t(c, K_STAGE0);
+        hip_check(m1::f32::launch_stage(m1::STAGE0, false, a, c->stream), "stage 0 kernel (f32)");
+    }
+    {
+        a.src = c->V1_32;
+        a.out = c->V2_32;
+        a.bnd = c->bnd32 + 4 * m2;
+        Timed t(c, K_STAGE1);
+        hip_check(m1::f32::launch_stage(m1::STAGE1, ifac, a, c->stream), "stage 1 kernel (f32)");
+    }
+    {
+        a.src = c->V2_32;
+        a.out = nullptr;
+        a.dose_hi = dose ? c->dose_hi : nullptr;
+        a.dose_lo = dose ? c->dose_lo : nullptr;
+        a.edep = edep ? c->edep32 : nullptr;
+        a.bnd = c->bnd32 + 8 * m2;
+        Timed t(c, K_STAGE2);
+        hip_check(m1::f32::launch_stage(m1::STAGE2, ifac, a, c->stream), "stage 2 kernel (f32)");
+    }
+}
+
+void forward_euler_f32(m1_ctx* c, const Num& n, bool dose, bool edep)
+{
+    {
+        m1::BoundaryArgs32 b{dims(c), c->spatial, c->ids, c->d_step, c->bnd32, 0};
+        Timed t(c, K_BOUNDARY);
+        hip_check(m1::f32::launch_boundary(b, c->stream), "boundary kernel (f32)");
+    }
+    m1::StageArgs32 a = base_args32(c, n);
+    a.src = c->U32;
+    a.U = c->U32;
+    a.out = c->work32;
+    a.dose_hi = dose ? c->dose_hi : nullptr;
+    a.dose_lo = dose ? c->dose_lo : nullptr;
+    a.edep = edep ? c->edep32 : nullptr;
+    a.bnd = c->bnd32;
+    {
+        Timed t(c, K_FE);
+        hip_check(m1::f32::launch_stage(m1::STAGE_FE, false, a, c->stream),
+                  "forward Euler kernel (f32)");
+    }
+    std::swap(c->U32, c->work32);
+}
+
+void run_scale_f32(m1_ctx* c, const Num& n)
+{
+    Timed t(c, K_SCALE);
+    hip_check(m1::f32::launch_scale(c->N, c->U32, c->ids, c->d_step, n.nuclear ? 1 : 0,
+                                    n.scattering != M1_SCATTERING_OFF ? 1 : 0, c->stream),
+              "source half-step kernel (f32)");
+}
+
+void step_f32(m1_ctx* c, const Num& n, int kind, double energy, double dE, bool dose, bool edep)
+{
+    if (dE <= 0.0 && kind != M1_STEP_FORWARD_EULER)
+        throw ValueError{"dE must be positive."};
+    const bool ifac = kind == M1_STEP_IF_SSPRK3 && has_sources(n);
+    const bool strang = kind == M1_STEP_STRANG;
+    fill_step(c, n, energy, dE, ifac, strang, *c->h_step);
+    upload_step32(c);
+    auto adv = [&] {
+        if (n.scheme == M1_FORWARD_EULER)
+            forward_euler_f32(c, n, dose, edep);
+        else
+            rk3_f32(c, n, false, dose, edep);
+    };
+    switch (kind) {
+    case M1_STEP_FORWARD_EULER:
+        forward_euler_f32(c, n, dose, edep);
+        break;
+    case M1_STEP_SSPRK3:
+        rk3_f32(c, n, false, dose, edep);
+        break;
+    case M1_STEP_IF_SSPRK3:
+        rk3_f32(c, n, ifac, dose, edep);
+        break;
+    case M1_STEP_STRANG:
+        if (has_sources(n))
+            run_scale_f32(c, n);
+        adv();
+        if (has_sources(n))
+            run_scale_f32(c, n);
+        break;
+    default:
+        throw ValueError{"unknown step kind"};
+    }
+}
+
+double cfl_step_f32(m1_ctx* c, const Num& n, double energy)
+{
+    StepCoef& s = *c->h_cfl;
+    std::memset(&s, 0, sizeof s);
+    for (size_t m = 0; m < c->mats.size(); ++m)
+        s.S0[m] = m1::stopping_power(c->mats[m], energy);
+    s.ed[0] = m1::energy_density(energy, c->beam_energy, c->energy_spread);
+    s.rf = std::exp(-(c->angular_spread * c->angular_spread));
+    set_norm(c, s);
+    hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, sizeof(StepCoef), hipMemcpyHostToDevice,
+                             c->stream),
+              "CFL coefficient upload");
+    hip_check(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream), "CFL reset");
+    m1::CflArgs32 a{dims(c), c->U32, c->ids, c->spatial, c->d_cfl, c->d_red, n.wave};
+    {
+        Timed t(c, K_CFL);
+        hip_check(m1::f32::launch_cfl(a, c->stream), "CFL kernel (f32)");
+    }
+    hip_check(hipMemcpyAsync(c->h_red, c->d_red, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream),
+              "CFL read-back");
+    sync(c);
+    double mx;
+    std::memcpy(&mx, c->h_red, sizeof mx);
+    if (!std::isfinite(mx))
+        throw NonFinite{"non-finite M1 state detected by the CFL reduction"};
+    const double dE = n.cfl / mx;
+    return dE <= n.max_step ? dE : n.max_step;
+}
+
+// Move the FP64 device state into the scaled float state and back.
+void narrow_state(m1_ctx* c)
+{
+    ensure_f32(c);
+    hip_check(m1::f32::launch_narrow_state(c->U, 1.0 / c->alpha, c->U32, 4 * c->N, c->stream),
+              "state narrowing");
+}
+
+void widen_state(m1_ctx* c, double* out)
+{
+    hip_check(m1::f32::launch_widen_state(c->U32, c->alpha, out, 4 * c->N, c->stream),
+              "state widening");
+}
+
 }  // namespace
 
 extern "C" {
@@ -635,6 +877,12 @@ int m1_set_problem(m1_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double dx, d
         ctx->beam_energy = beam->energy;
         ctx->energy_spread = beam->energy_spread;
         ctx->angular_spread = beam->angular_spread;
+        {
+            double pk = 0.0;
+            for (size_t i = 0; i < size_t(nx) * ny; ++i)
+                pk = std::max(pk, inflow_spatial[i]);
+            ctx->alpha = march_alpha(ctx, {pk});
+        }
         const size_t m2 = size_t(nx) * ny;
         dev_alloc(&ctx->U, 4 * size_t(N), "hipMalloc state");
         dev_alloc(&ctx->V1, 4 * size_t(N), "hipMalloc stage 1");
@@ -740,12 +988,20 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
         };
         record(energy);
         size_t ti = 0;
+        const bool f32 = n.precision == M1_FP32_BULK;
+        if (f32) {
+            narrow_state(ctx);
+            hip_check(hipMemsetAsync(ctx->dose_hi, 0, size_t(ctx->N) * sizeof(float), ctx->stream),
+                      "dose reset");
+            hip_check(hipMemsetAsync(ctx->dose_lo, 0, size_t(ctx->N) * sizeof(float), ctx->stream),
+                      "dose reset");
+        }
         const int kind = n.scheme == M1_IF_SSPRK3 ? M1_STEP_IF_SSPRK3
                          : has_sources(n)         ? M1_STEP_STRANG
                          : n.scheme == M1_FORWARD_EULER ? M1_STEP_FORWARD_EULER
                                                         : M1_STEP_SSPRK3;
         while (energy > emin) {
-            double dE = cfl_step(ctx, n, energy);
+            double dE = f32 ? cfl_step_f32(ctx, n, energy) : cfl_step(ctx, n, energy);
             if (ti < targets.size()) {
                 const double t = targets[ti];
                 if (energy > t && t >= energy - dE)
@@ -753,13 +1009,18 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
             }
             if (energy - emin < dE)
                 dE = energy - emin;
-            step(ctx, n, kind, energy, dE, ctx->dose, nullptr);
+            if (f32)
+                step_f32(ctx, n, kind, energy, dE, true, false);
+            else
+                step(ctx, n, kind, energy, dE, ctx->dose, nullptr);
             energy -= dE;
             record(energy);
             if (ti < targets.size()) {
                 const double t = targets[ti];
                 if (std::fabs(energy - t) <= 1e-12) {
                     if (snapshots) {
+                        if (f32)
+                            widen_state(ctx, ctx->work);
                         // slot of the first request equal to this target
                         int slot = 0;
                         for (int i = 0; i < n_snapshots; ++i)
@@ -768,7 +1029,8 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
                                 break;
                             }
                         hip_check(hipMemcpyAsync(snapshots + size_t(slot) * 4 * size_t(ctx->N),
-                                                 ctx->U, 4 * size_t(ctx->N) * sizeof(double),
+                                                 f32 ? ctx->work : ctx->U,
+                                                 4 * size_t(ctx->N) * sizeof(double),
                                                  hipMemcpyDeviceToHost, ctx->stream),
                                   "snapshot download");
                         if (snapshot_taken)
@@ -777,6 +1039,12 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
                     ++ti;
                 }
             }
+        }
+        if (f32) {
+            widen_state(ctx, ctx->U);
+            hip_check(m1::f32::launch_finalize_dose(ctx->dose_hi, ctx->dose_lo, ctx->dose, ctx->N,
+                                                    ctx->stream),
+                      "dose finalisation");
         }
         sync(ctx);
         if (n_energies)
@@ -793,23 +1061,45 @@ int m1_spatial_operator(m1_ctx* ctx, const m1_numerics* num, double energy,
         if (!rhs)
             throw ValueError{"rhs must not be NULL"};
         const size_t m2 = size_t(ctx->nx) * ctx->ny;
-        if (boundary_state) {
+        const bool f32 = n.precision == M1_FP32_BULK;
+        if (boundary_state)
             hip_check(hipMemcpyAsync(ctx->bnd, boundary_state, 4 * m2 * sizeof(double),
                                      hipMemcpyHostToDevice, ctx->stream),
                       "boundary upload");
-            fill_step(ctx, n, energy, 0.0, false, false, *ctx->h_step);
+        fill_step(ctx, n, energy, 0.0, false, false, *ctx->h_step);
+        if (f32) {
+            narrow_state(ctx);
+            upload_step32(ctx);
+            if (boundary_state) {
+                hip_check(m1::f32::launch_narrow_state(ctx->bnd, 1.0 / ctx->alpha, ctx->bnd32,
+                                                       long(4 * m2), ctx->stream),
+                          "boundary narrowing");
+            } else {
+                m1::BoundaryArgs32 b{dims(ctx), ctx->spatial, ctx->ids, ctx->d_step, ctx->bnd32, 0};
+                hip_check(m1::f32::launch_boundary(b, ctx->stream), "boundary kernel (f32)");
+            }
+            m1::StageArgs32 a = base_args32(ctx, n);
+            a.src = ctx->U32;
+            a.U = ctx->U32;
+            a.out = ctx->work32;
+            a.bnd = ctx->bnd32;
+            {
+                Timed t(ctx, K_RHS);
+                hip_check(m1::f32::launch_stage(m1::STAGE_RHS, false, a, ctx->stream),
+                          "spatial operator kernel (f32)");
+            }
+            hip_check(m1::f32::launch_widen_state(ctx->work32, ctx->alpha, ctx->work, 4 * ctx->N,
+                                                  ctx->stream),
+                      "rhs widening");
         } else {
-            fill_step(ctx, n, energy, 0.0, false, false, *ctx->h_step);
-        }
-        upload_step(ctx);
-        if (!boundary_state)
-            run_boundary(ctx, n, false);
-        m1::StageArgs a = base_args(ctx, n);
-        a.src = ctx->U;
-        a.U = ctx->U;
-        a.out = ctx->work;
-        a.bnd = ctx->bnd;
-        {
+            upload_step(ctx);
+            if (!boundary_state)
+                run_boundary(ctx, n, false);
+            m1::StageArgs a = base_args(ctx, n);
+            a.src = ctx->U;
+            a.U = ctx->U;
+            a.out = ctx->work;
+            a.bnd = ctx->bnd;
             Timed t(ctx, K_RHS);
             hip_check(stage(ctx, n, m1::STAGE_RHS, false, a), "spatial operator kernel");
         }
@@ -825,7 +1115,13 @@ int m1_compute_energy_step(m1_ctx* ctx, const m1_numerics* num, double energy, d
     return guarded(ctx, [&] {
         require_problem(ctx);
         const Num n = read_numerics(num);
-        const double d = cfl_step(ctx, n, energy);
+        double d;
+        if (n.precision == M1_FP32_BULK) {
+            narrow_state(ctx);
+            d = cfl_step_f32(ctx, n, energy);
+        } else {
+            d = cfl_step(ctx, n, energy);
+        }
         if (dE)
             *dE = d;
     });
@@ -837,7 +1133,17 @@ int m1_step(m1_ctx* ctx, const m1_numerics* num, int32_t kind, double energy, do
     return guarded(ctx, [&] {
         require_problem(ctx);
         const Num n = read_numerics(num);
-        step(ctx, n, kind, energy, dE, nullptr, edep ? ctx->edep : nullptr);
+        if (n.precision == M1_FP32_BULK) {
+            narrow_state(ctx);
+            step_f32(ctx, n, kind, energy, dE, false, edep != nullptr);
+            widen_state(ctx, ctx->U);
+            if (edep)
+                hip_check(m1::f32::launch_widen_state(ctx->edep32, ctx->alpha, ctx->edep, ctx->N,
+                                                      ctx->stream),
+                          "deposition widening");
+        } else {
+            step(ctx, n, kind, energy, dE, nullptr, edep ? ctx->edep : nullptr);
+        }
         if (edep)
             hip_check(hipMemcpyAsync(edep, ctx->edep, size_t(ctx->N) * sizeof(double),
                                      hipMemcpyDeviceToHost, ctx->stream),

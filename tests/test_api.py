@@ -145,15 +145,15 @@ def test_gamma_identical_and_shifted_doses():
     rng = np.random.default_rng(0)
     ref = np.zeros((8, 8, 30))
     z = np.arange(30)
-    ref[:, :, :] = np.exp(-((z - 15) / 5.0) ** 2)
+    ref[:, :, :] = np.clip(np.minimum(z - 2.0, 27.0 - z) / 5.0, 0.0, 1.0)  # flat top
     assert metrics.gamma_pass_rate(ref, ref, (1e-3, 1e-3, 1e-3)) == 100.0
-    # a 2 % dose error everywhere fails 1%/1mm in the flat high-dose region
-    # but passes 3%/3mm
+    # a 2 % dose error everywhere fails 1%/1mm on the plateau (no nearby
+    # point has the right dose) but passes 3%/3mm
     assert metrics.gamma_pass_rate(1.02 * ref, ref, (1e-3,) * 3, 1.0, 1.0) < 100.0
     assert metrics.gamma_pass_rate(1.02 * ref, ref, (1e-3,) * 3, 3.0, 3.0) == 100.0
     # a half-voxel shift along z passes 1 mm DTA
     shifted = np.zeros_like(ref)
-    shifted[:, :, :] = np.exp(-((z - 15.5) / 5.0) ** 2)
+    shifted[:, :, :] = np.clip(np.minimum(z - 2.5, 26.5 - z) / 5.0, 0.0, 1.0)
     assert metrics.gamma_pass_rate(shifted, ref, (1e-3,) * 3, 1.0, 1.0) == 100.0
 
 

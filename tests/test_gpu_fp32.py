@@ -1,0 +1,101 @@
+"""FP32-bulk march (Precision.FP32_BULK) against the FP64 oracle, on an MI355X.
+
+FP32-bulk runs reconstruction, fluxes and stage algebra in float on the state
+normalised by the peak inflow fluence and keeps closure, eigenvalues, HLL,
+projection, ghost extrapolation and the dose accumulation in double (or a
+compensated float pair).  Acceptance, as stated for the mixed-precision
+march: maximum-dose error <= 0.5 % and gamma(1 %/1 mm) pass rate >= 99.5 %
+against the FP64 oracle.  Per operator the float path is compared with
+tolerances written below.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import config_from_fixture, thesis_cases
+from proton_solver import metrics
+from proton_solver import _native
+from test_gpu_parity import CONFIG1, HETERO, STEP_KINDS, random_state, say, setup, small_config
+from proton_solver import Box, Material, Physics, Scattering
+
+pytestmark = pytest.mark.gpu
+
+F32 = _native.FP32_BULK
+MAX_DOSE_TOL = 5e-3
+GAMMA_MIN = 99.5
+
+
+def test_fp32_spatial_operator_close_to_oracle():
+    cfg = small_config(shape=(11, 13, 37), objects=HETERO)
+    grid, mm, inflow, ctx, orc = setup(cfg)
+    num = _native.numerics_struct(cfg, F32)
+    for seed in (1, 2):
+        U = random_state(grid.shape, seed)
+        ctx.set_state(U)
+        rhs = ctx.spatial_operator(num, 101.3)
+        ref = orc.spatial_operator(U, 101.3)
+        err = np.abs(rhs - ref).max() / np.abs(ref).max()
+        say("fp32 spatial operator seed", seed, "max|d|/max", err)
+        # float WENO / divergence on O(1) normalised values
+        assert err < 1e-4
+
+
+def test_fp32_cfl_close_to_oracle():
+    cfg = small_config(objects=HETERO)
+    grid, mm, inflow, ctx, orc = setup(cfg)
+    num = _native.numerics_struct(cfg, F32)
+    U = random_state(grid.shape, 3)
+    ctx.set_state(U)
+    got = ctx.compute_energy_step(num, 104.0)
+    ref = orc.compute_energy_step(U, 104.0)
+    say("fp32 cfl", got, ref)
+    assert got == pytest.approx(ref, rel=1e-5)
+
+
+@pytest.mark.parametrize("kind", list(STEP_KINDS))
+def test_fp32_single_step_close_to_oracle(kind):
+    bone = Box(-0.004, 0.0, -0.005, 0.005, 0.004, 0.010, Material.BONE_COMPACT)
+    cfg = small_config(shape=(9, 10, 20), physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
+                       objects=(bone,))
+    grid, mm, inflow, ctx, orc = setup(cfg)
+    num = _native.numerics_struct(cfg, F32)
+    U = random_state(grid.shape, 7, scale=1e8)
+    ctx.set_state(U)
+    edep = ctx.step(num, STEP_KINDS[kind], 99.0, 0.37)
+    U_gpu = ctx.get_state()
+    U_ref, edep_ref = orc.step(U, 99.0, 0.37, kind)
+    eu = np.abs(U_gpu - U_ref).max() / np.abs(U_ref).max()
+    ee = np.abs(edep - edep_ref).max() / np.abs(edep_ref).max()
+    say("fp32 step", kind, "state err", eu, "edep err", ee)
+    assert eu < 1e-4 and ee < 1e-4
+
+
+def _accuracy(cfg, label):
+    grid, mm, inflow, ctx, orc = setup(cfg)
+    num = _native.numerics_struct(cfg, F32)
+    E, _ = ctx.march(num)
+    dose = ctx.get_dose()
+    U = ctx.get_state()
+    _, dose_ref, E_ref, _ = orc.march()
+    md = metrics.maximum_dose_relative_difference(dose, dose_ref)
+    mx = metrics.max_abs_difference_over_max(dose, dose_ref)
+    gam = metrics.gamma_pass_rate(dose, dose_ref, (grid.dx, grid.dy, grid.dz), 1.0, 1.0, 10.0)
+    say(label, "steps", len(E) - 1, "vs", len(E_ref) - 1, "max-dose err", md,
+        "max|dD|/maxD", mx, "gamma 1%/1mm", gam)
+    assert np.all(np.isfinite(U)) and np.all(dose >= 0)
+    return md, mx, gam, dose
+
+
+def test_fp32_config1_march_accuracy():
+    md, mx, gam, _ = _accuracy(CONFIG1, "config1 fp32")
+    assert abs(md) <= MAX_DOSE_TOL
+    assert gam >= GAMMA_MIN
+
+
+@pytest.mark.parametrize("idx", range(6))
+def test_fp32_golden_cases_accuracy(idx):
+    case = [c for c in thesis_cases() if "coarse_grid_2mm" in c["source"]][idx]
+    md, mx, gam, dose = _accuracy(config_from_fixture(case["config"]), case["source"])
+    assert abs(md) <= MAX_DOSE_TOL
+    assert gam >= GAMMA_MIN
+    assert abs(dose.max() / case["maximum_dose_Gy"] - 1) <= MAX_DOSE_TOL
