@@ -165,6 +165,7 @@ def main():
         steps_per_march.append(len(E) - 1)
     elapsed = time.perf_counter() - t0
     stats = ctx.kernel_stats()
+    tiles_eval, tiles_full = ctx.tile_stats()
     ctx.set_profiling(False)
     dose = ctx.get_dose()
 
@@ -173,14 +174,17 @@ def main():
     solves_per_s = args.steps / elapsed
     cell_steps = n_cells * sum(steps_per_march)
 
-    # Dominant stage kernel and its HBM roofline.
+    # Dominant stage kernel and its HBM roofline.  Cells per launch are the
+    # cells of the evaluated tiles (all cells when no tile is skipped).
+    eval_frac = (tiles_eval / tiles_full) if tiles_full else 1.0
+    cells_per_launch = n_cells * eval_frac
     best = None
     for name, per_cell in stage_bytes.items():
         launches, total_ms = stats.get(name, (0, 0.0))
         if launches == 0:
             continue
         avg_s = total_ms / launches / 1000.0
-        achieved = per_cell * n_cells / avg_s / 1e9
+        achieved = per_cell * cells_per_launch / avg_s / 1e9
         if best is None or total_ms > best[1]:
             best = (name, total_ms, achieved, avg_s, per_cell)
     roofline = None
@@ -191,6 +195,7 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
             "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
             "algorithmic_bytes_per_cell": per_cell,
+            "cells_per_launch": round(cells_per_launch),
         }
 
     accuracy = None
@@ -236,6 +241,8 @@ def main():
             "max_dose_Gy": float(dose.max()),
         },
         "cell_steps_per_s": cell_steps / elapsed,
+        "cell_steps_per_s_evaluated": cell_steps * eval_frac / elapsed,
+        "tiles_evaluated_fraction": eval_frac,
         "kernel_ms_per_solve": {k: round(v[1] / max(args.steps, 1), 3) for k, v in stats.items()
                                 if v[0]},
         "accuracy_vs_fp64": accuracy,

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define M1_ABI_VERSION 1
+#define M1_ABI_VERSION 2
 
 enum {
     M1_OK = 0,
@@ -103,6 +103,12 @@ typedef struct {
     double cfl;
     double energy_min;                  /* MeV */
     double energy_window_sigma;
+    /* FP32_BULK only: workgroup tiles (16 x 8 x 32 cells) whose 9-cell
+     * neighbourhood holds phi below tile_threshold x (peak inflow fluence)
+     * are not evaluated and are kept as exact vacuum; 0 evaluates every
+     * tile.  No reference counterpart (the reference's closest analogue is
+     * its moving active domain, moving_active_domain.py:107-193). */
+    double tile_threshold;
 } m1_numerics;
 
 /* Library ABI version (M1_ABI_VERSION). */
@@ -175,6 +181,9 @@ const char *m1_kernel_name(int32_t kernel);
 int m1_kernel_stats(m1_ctx *ctx, int32_t kernel, int64_t *launches,
                     double *total_ms);
 int m1_reset_stats(m1_ctx *ctx);
+/* Tiles evaluated by the stage kernels since the last m1_reset_stats, and the
+ * number a full-domain march would have evaluated. */
+int m1_tile_stats(m1_ctx *ctx, int64_t *evaluated, int64_t *full);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,33 @@ struct StageArgs32 {
     const float* bnd;       // (4, nx, ny) scaled boundary state of this stage
     const StepCoef32* coef;
     int wave_mode;
+    // Tile skipping (nullable): compute[t] != 0 for workgroup tiles to
+    // evaluate; STAGE2 writes the tile's max phi to tile_max[t].
+    const uint8_t* compute;
+    float* tile_max;
+};
+
+// Workgroup tile geometry of the FP32 stage kernels (cells per tile in x, y, z).
+constexpr int F32_TX = 16, F32_TY = 8, F32_TZ = 32;
+
+struct TileMaskArgs {
+    int ntx, nty, ntz;       // tiles per axis
+    const float* tile_max;   // max phi per tile after this step (valid where computed)
+    const uint8_t* computed; // tiles evaluated this step
+    uint8_t* next;           // tiles to evaluate next step
+    float threshold;         // activity threshold on the normalised phi
+    int inflow_on;           // z = 0 tiles fed by the inflow
+    int rx, ry, rz;          // dilation radius in tiles
+    int* n_active;           // count of next-step tiles (atomic)
+};
+
+struct TileZeroArgs {
+    GridDims g;
+    const uint8_t* computed;
+    const uint8_t* next;
+    float* U;
+    float* V1;
+    float* V2;
 };
 
 struct CflArgs32 {
@@ -161,6 +188,8 @@ hipError_t launch_narrow_state(const double* U, double inv_alpha, float* out, lo
                                hipStream_t s);
 hipError_t launch_scale(long n, float* U, const uint8_t* ids, const StepCoef* coef,
                         int nuclear, int scattering, hipStream_t s);
+hipError_t launch_tile_mask(const TileMaskArgs& a, hipStream_t s);
+hipError_t launch_tile_zero(const TileZeroArgs& a, hipStream_t s);
 }  // namespace f32
 
 #define M1_DECLARE_LAUNCHERS(NS)                                                          \

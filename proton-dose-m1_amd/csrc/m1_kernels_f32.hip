@@ -23,9 +23,9 @@ namespace m1 {
 namespace f32 {
 namespace {
 
-constexpr int TY = 8;
-constexpr int TZ = 32;
-constexpr int TX = 16;
+constexpr int TY = F32_TY;
+constexpr int TZ = F32_TZ;
+constexpr int TX = F32_TX;
 constexpr int NT = TY * TZ;
 constexpr int PY = TY + 6;
 constexpr int PZ = TZ + 6;
@@ -246,8 +246,13 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
     __shared__ float P[4][PY][PZ];
     __shared__ float Fy[4][TY + 1][TZ];
     __shared__ float Fz[4][TY][TZ + 1];
+    __shared__ float wmax[NT / 64];
 
+    const long tile = (long(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (a.compute && !a.compute[tile])
+        return;  // whole tile and its 9-cell neighbourhood are vacuum
     const int tid = threadIdx.x;
+    float phimax = 0.f;
     {
         const unsigned* src = reinterpret_cast<const unsigned*>(a.coef);
         unsigned* dst = reinterpret_cast<unsigned*>(&sc);
@@ -445,6 +450,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     a.U[i + c * N] = u[c];
+                phimax = fmaxf(phimax, u[0]);
                 const float q0 = un[0] * S0;
                 const float q1 = fmaxf(a.V1[i] * sc.g1p[m], 0.f) * sc.S1[m];
                 const float q2 = W[2][0] * S2;
@@ -463,6 +469,21 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 W[k][c] = W[k + 1][c];
+    }
+    if constexpr (KIND == STAGE2) {
+        if (a.tile_max) {
+            for (int off = 32; off > 0; off >>= 1)
+                phimax = fmaxf(phimax, __shfl_xor(phimax, off, 64));
+            if ((tid & 63) == 0)
+                wmax[tid >> 6] = phimax;
+            __syncthreads();
+            if (tid == 0) {
+                float m = wmax[0];
+                for (int w = 1; w < NT / 64; ++w)
+                    m = fmaxf(m, wmax[w]);
+                a.tile_max[tile] = m;
+            }
+        }
     }
 }
 
@@ -633,7 +654,77 @@ __global__ void scale_kernel(long n, float* U, const uint8_t* ids, const StepCoe
     }
 }
 
+// Next-step tile set: a tile is evaluated when any tile within the dilation
+// radius holds phi above the threshold after this step, or when it is a
+// z = 0 tile while the inflow spectrum is on.
+__global__ void tile_mask_kernel(TileMaskArgs a)
+{
+    const int n = a.ntx * a.nty * a.ntz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int tz = t % a.ntz, ty = (t / a.ntz) % a.nty, tx = t / (a.ntz * a.nty);
+    bool on = a.inflow_on && tz == 0;
+    for (int dx = -a.rx; dx <= a.rx && !on; ++dx) {
+        const int xx = tx + dx;
+        if (xx < 0 || xx >= a.ntx)
+            continue;
+        for (int dy = -a.ry; dy <= a.ry && !on; ++dy) {
+            const int yy = ty + dy;
+            if (yy < 0 || yy >= a.nty)
+                continue;
+            for (int dz = -a.rz; dz <= a.rz && !on; ++dz) {
+                const int zz = tz + dz;
+                if (zz < 0 || zz >= a.ntz)
+                    continue;
+                const int u = (xx * a.nty + yy) * a.ntz + zz;
+                on = a.computed[u] && a.tile_max[u] > a.threshold;
+            }
+        }
+    }
+    a.next[t] = on ? 1 : 0;
+    if (on)
+        atomicAdd(a.n_active, 1);
+}
+
+// Tiles that leave the evaluated set become exact vacuum (U = V1 = V2 = 0).
+__global__ void tile_zero_kernel(TileZeroArgs a)
+{
+    const long t = (long(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (!(a.computed[t] && !a.next[t]))
+        return;
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const long N = long(nx) * ny * nz;
+    const int z = blockIdx.x * TZ + threadIdx.x % TZ;
+    const int y = blockIdx.y * TY + threadIdx.x / TZ;
+    if (z >= nz || y >= ny)
+        return;
+    for (int x = blockIdx.z * TX; x < min(int(blockIdx.z) * TX + TX, nx); ++x) {
+        const long i = (long(x) * ny + y) * nz + z;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            a.U[i + c * N] = 0.f;
+            a.V1[i + c * N] = 0.f;
+            a.V2[i + c * N] = 0.f;
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_tile_mask(const TileMaskArgs& a, hipStream_t s)
+{
+    const int n = a.ntx * a.nty * a.ntz;
+    hipLaunchKernelGGL(tile_mask_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_zero(const TileZeroArgs& a, hipStream_t s)
+{
+    const dim3 grid((a.g.nz + TZ - 1) / TZ, (a.g.ny + TY - 1) / TY, (a.g.nx + TX - 1) / TX);
+    hipLaunchKernelGGL(tile_zero_kernel, grid, dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_stage(int kind, bool ifac, const StageArgs32& a, hipStream_t s)
 {

@@ -73,6 +73,14 @@ struct m1_ctx {
     float *U32 = nullptr, *V1_32 = nullptr, *V2_32 = nullptr, *work32 = nullptr,
           *dose_hi = nullptr, *dose_lo = nullptr, *edep32 = nullptr, *bnd32 = nullptr;
     double alpha = 1.0;
+    // tile skipping (FP32 bulk)
+    uint8_t* d_compute[2] = {nullptr, nullptr};
+    float* d_tile_max = nullptr;
+    int* d_nactive = nullptr;
+    int ntx = 0, nty = 0, ntz = 0;
+    int cur = 0;
+    bool skip_all_computed = true;  // first step of a march evaluates every tile
+    int64_t tiles_evaluated = 0, tiles_full = 0;
     m1::StepCoef32* d_step32 = nullptr;
     m1::StepCoef32* h_step32 = nullptr;
     StepCoef* d_step = nullptr;
@@ -80,6 +88,8 @@ struct m1_ctx {
     unsigned long long* d_red = nullptr;
     StepCoef* h_step = nullptr;
     StepCoef* h_cfl = nullptr;
+    int* h_nactive = nullptr;
+    int pending_tiles = 0;
     unsigned long long* h_red = nullptr;
 
     bool prof = false;
@@ -101,11 +111,15 @@ void m1_ctx::free_problem()
     for (void* p : {(void*)U, (void*)V1, (void*)V2, (void*)work, (void*)dose, (void*)edep,
                     (void*)bnd, (void*)spatial, (void*)ids, (void*)U32, (void*)V1_32,
                     (void*)V2_32, (void*)work32, (void*)dose_hi, (void*)dose_lo, (void*)edep32,
-                    (void*)bnd32})
+                    (void*)bnd32, (void*)d_compute[0], (void*)d_compute[1], (void*)d_tile_max,
+                    (void*)d_nactive})
         if (p)
             (void)hipFree(p);
     U = V1 = V2 = work = dose = edep = bnd = spatial = nullptr;
     U32 = V1_32 = V2_32 = work32 = dose_hi = dose_lo = edep32 = bnd32 = nullptr;
+    d_compute[0] = d_compute[1] = nullptr;
+    d_tile_max = nullptr;
+    d_nactive = nullptr;
     ids = nullptr;
     have_problem = false;
 }
@@ -119,7 +133,7 @@ m1_ctx::~m1_ctx()
     for (void* p : {(void*)d_step, (void*)d_cfl, (void*)d_red, (void*)d_step32})
         if (p)
             (void)hipFree(p);
-    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red, (void*)h_step32})
+    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red, (void*)h_step32, (void*)h_nactive})
         if (p)
             (void)hipHostFree(p);
     for (auto& e : pool)
@@ -193,7 +207,7 @@ void sync(m1_ctx* c)
 // ------------------------------------------------------------ numerics
 struct Num {
     int scheme, wave, scattering, nuclear, interp, precision;
-    double max_step, cfl, emin, window;
+    double max_step, cfl, emin, window, tile_threshold;
 };
 
 Num read_numerics(const m1_numerics* n)
@@ -202,7 +216,7 @@ Num read_numerics(const m1_numerics* n)
         throw ValueError{"numerics must not be NULL"};
     Num r{n->energy_scheme, n->wave_speeds, n->scattering, n->nuclear,
           n->coefficient_interpolation, n->precision, n->max_energy_step, n->cfl,
-          n->energy_min, n->energy_window_sigma};
+          n->energy_min, n->energy_window_sigma, n->tile_threshold};
     if (r.scheme < M1_FORWARD_EULER || r.scheme > M1_IF_SSPRK3)
         throw NotImplemented{"Energy scheme has not been migrated yet."};
     if (r.wave != M1_WAVE_EXACT && r.wave != M1_WAVE_UNIT)
@@ -222,6 +236,8 @@ Num read_numerics(const m1_numerics* n)
         throw ValueError{"cfl must lie in (0, 1]."};
     if (r.emin < 0)
         throw ValueError{"energy_min cannot be negative."};
+    if (!(r.tile_threshold >= 0.0 && r.tile_threshold < 1.0))
+        throw ValueError{"tile_threshold must lie in [0, 1)."};
     return r;
 }
 
@@ -565,6 +581,14 @@ void ensure_f32(m1_ctx* c)
     dev_alloc(&c->dose_lo, N, "hipMalloc dose (f32)");
     dev_alloc(&c->edep32, N, "hipMalloc deposition (f32)");
     dev_alloc(&c->bnd32, 12 * m2, "hipMalloc boundary (f32)");
+    c->ntx = (c->nx + m1::F32_TX - 1) / m1::F32_TX;
+    c->nty = (c->ny + m1::F32_TY - 1) / m1::F32_TY;
+    c->ntz = (c->nz + m1::F32_TZ - 1) / m1::F32_TZ;
+    const size_t nt = size_t(c->ntx) * c->nty * c->ntz;
+    dev_alloc(&c->d_compute[0], nt, "hipMalloc tile set");
+    dev_alloc(&c->d_compute[1], nt, "hipMalloc tile set");
+    dev_alloc(&c->d_tile_max, nt, "hipMalloc tile max");
+    dev_alloc(&c->d_nactive, 1, "hipMalloc tile count");
     if (!c->d_step32) {
         dev_alloc(&c->d_step32, 1, "hipMalloc coefficients (f32)");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_step32), sizeof(m1::StepCoef32)),
@@ -635,6 +659,14 @@ m1::StageArgs32 base_args32(const m1_ctx* c, const Num& n)
     return a;
 }
 
+// Tile set of this step (nullptr: evaluate every tile).
+const uint8_t* step_tiles(const m1_ctx* c, const Num& n)
+{
+    if (n.tile_threshold <= 0.0 || c->skip_all_computed)
+        return nullptr;
+    return c->d_compute[c->cur];
+}
+
 void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
 {
     {
@@ -646,6 +678,8 @@ void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
     m1::StageArgs32 a = base_args32(c, n);
     a.U = c->U32;
     a.V1 = c->V1_32;
+    a.compute = step_tiles(c, n);
+    a.tile_max = n.tile_threshold > 0.0 ? c->d_tile_max : nullptr;
     {
         a.src = c->U32;
         a.out = c->V1_32;
@@ -761,6 +795,10 @@ double cfl_step_f32(m1_ctx* c, const Num& n, double energy)
                              hipMemcpyDeviceToHost, c->stream),
               "CFL read-back");
     sync(c);
+    if (c->pending_tiles) {
+        c->tiles_evaluated += *c->h_nactive;
+        c->pending_tiles = 0;
+    }
     double mx;
     std::memcpy(&mx, c->h_red, sizeof mx);
     if (!std::isfinite(mx))
@@ -769,10 +807,47 @@ double cfl_step_f32(m1_ctx* c, const Num& n, double energy)
     return dE <= n.max_step ? dE : n.max_step;
 }
 
+// Next step's tile set from this step's tile maxima (rk3 steps only; the
+// forward-Euler and Strang paths evaluate every tile).
+void update_tiles(m1_ctx* c, const Num& n, double energy_next)
+{
+    const int64_t nt = int64_t(c->ntx) * c->nty * c->ntz;
+    c->tiles_full += nt;
+    c->tiles_evaluated += c->skip_all_computed ? nt : 0;
+    if (n.tile_threshold <= 0.0)
+        return;
+    if (c->skip_all_computed) {
+        // the first step evaluated every tile: mark all as computed
+        hip_check(hipMemsetAsync(c->d_compute[c->cur], 1, size_t(nt), c->stream), "tile set");
+    }
+    // Inflow spectrum relative to its peak anywhere in the next two maximal steps.
+    const double lo = energy_next - 2.0 * n.max_step;
+    const double e = std::min(std::max(c->beam_energy, lo), energy_next);
+    const double d = (e - c->beam_energy) / c->energy_spread;
+    const int inflow_on = std::exp(-0.5 * d * d) > n.tile_threshold ? 1 : 0;
+    hip_check(hipMemsetAsync(c->d_nactive, 0, sizeof(int), c->stream), "tile count reset");
+    m1::TileMaskArgs ma{c->ntx, c->nty, c->ntz, c->d_tile_max, c->d_compute[c->cur],
+                        c->d_compute[1 - c->cur], float(n.tile_threshold), inflow_on,
+                        1, 2, 1, c->d_nactive};
+    hip_check(m1::f32::launch_tile_mask(ma, c->stream), "tile mask kernel");
+    m1::TileZeroArgs za{dims(c), c->d_compute[c->cur], c->d_compute[1 - c->cur], c->U32,
+                        c->V1_32, c->V2_32};
+    hip_check(m1::f32::launch_tile_zero(za, c->stream), "tile zero kernel");
+    hip_check(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream),
+              "tile count read-back");
+    c->pending_tiles = 1;
+    c->cur = 1 - c->cur;
+    c->skip_all_computed = false;
+}
+
 // Move the FP64 device state into the scaled float state and back.
 void narrow_state(m1_ctx* c)
 {
     ensure_f32(c);
+    c->skip_all_computed = true;
+    c->cur = 0;
+    c->pending_tiles = 0;
     hip_check(m1::f32::launch_narrow_state(c->U, 1.0 / c->alpha, c->U32, 4 * c->N, c->stream),
               "state narrowing");
 }
@@ -818,6 +893,8 @@ int m1_create(m1_ctx** out, int device)
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_cfl), sizeof(StepCoef)),
                   "hipHostMalloc");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_red), sizeof(unsigned long long)),
+                  "hipHostMalloc");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_nactive), sizeof(int)),
                   "hipHostMalloc");
     } catch (const HipError& e) {
         g_create_error = e.msg;
@@ -1009,10 +1086,13 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
             }
             if (energy - emin < dE)
                 dE = energy - emin;
-            if (f32)
+            if (f32) {
                 step_f32(ctx, n, kind, energy, dE, true, false);
-            else
+                if (kind == M1_STEP_IF_SSPRK3 || kind == M1_STEP_SSPRK3)
+                    update_tiles(ctx, n, energy - dE);
+            } else {
                 step(ctx, n, kind, energy, dE, ctx->dose, nullptr);
+            }
             energy -= dE;
             record(energy);
             if (ti < targets.size()) {
@@ -1180,10 +1260,22 @@ int m1_kernel_stats(m1_ctx* ctx, int32_t kernel, int64_t* launches, double* tota
     });
 }
 
+int m1_tile_stats(m1_ctx* ctx, int64_t* evaluated, int64_t* full)
+{
+    return guarded(ctx, [&] {
+        if (evaluated)
+            *evaluated = ctx->tiles_evaluated;
+        if (full)
+            *full = ctx->tiles_full;
+    });
+}
+
 int m1_reset_stats(m1_ctx* ctx)
 {
     return guarded(ctx, [&] {
         sync(ctx);
+        ctx->tiles_evaluated = 0;
+        ctx->tiles_full = 0;
         for (int k = 0; k < K_COUNT; ++k) {
             ctx->launches[k] = 0;
             ctx->total_ms[k] = 0.0;

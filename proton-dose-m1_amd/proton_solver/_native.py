@@ -16,7 +16,7 @@ import numpy as np
 from .enums import CoefficientInterpolation, EnergyScheme, Precision, Scattering, WaveSpeeds
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libm1march.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 M1_OK = 0
 M1_ERR_VALUE = -1
@@ -53,7 +53,8 @@ class m1_numerics(C.Structure):
                 ("scattering", C.c_int32), ("nuclear", C.c_int32),
                 ("coefficient_interpolation", C.c_int32), ("precision", C.c_int32),
                 ("max_energy_step", C.c_double), ("cfl", C.c_double),
-                ("energy_min", C.c_double), ("energy_window_sigma", C.c_double)]
+                ("energy_min", C.c_double), ("energy_window_sigma", C.c_double),
+                ("tile_threshold", C.c_double)]
 
 
 # (name, restype, argtypes) of every exported function.
@@ -81,6 +82,7 @@ SIGNATURES = [
     ("m1_kernel_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
     ("m1_reset_stats", C.c_int, [C.c_void_p]),
+    ("m1_tile_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 ]
 
 _lib = None
@@ -135,10 +137,11 @@ def numerics_struct(config, precision_code: int | None = None) -> m1_numerics:
         raise NotImplementedError(f"Energy scheme {n.energy_scheme} has not been migrated yet.")
     if precision_code is None:
         precision_code = PRECISION_CODE[config.device.precision]
+    tile = float(getattr(getattr(config, "device", None), "tile_threshold", 0.0))
     return m1_numerics(_SCHEME[n.energy_scheme], _WAVE[n.wave_speeds], _SCAT[ph.scattering],
                        int(bool(ph.nuclear)), _INTERP[ph.coefficient_interpolation],
                        int(precision_code), float(n.max_energy_step), float(n.cfl),
-                       float(n.energy_min), float(n.energy_window_sigma))
+                       float(n.energy_min), float(n.energy_window_sigma), tile)
 
 
 def _table(E, v, slope, keep):
@@ -292,6 +295,13 @@ class Context:
 
     def reset_stats(self):
         self._check(self.lib.m1_reset_stats(self.h))
+
+    def tile_stats(self):
+        """(tiles evaluated, tiles a full-domain march evaluates) since reset_stats."""
+        a = C.c_int64()
+        b = C.c_int64()
+        self._check(self.lib.m1_tile_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def kernel_stats(self):
         out = {}

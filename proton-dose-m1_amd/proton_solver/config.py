@@ -136,12 +136,17 @@ class DeviceOptions:
 
     precision: Precision = Precision.FP64
     device: int = 0
+    # FP32_BULK only: skip workgroup tiles whose neighbourhood holds phi below
+    # this fraction of the peak inflow fluence (0 evaluates every tile).
+    tile_threshold: float = 1.0e-10
 
     def __post_init__(self):
         if not isinstance(self.precision, Precision):
             raise ValueError("precision must be a Precision value.")
         if type(self.device) is not int or self.device < 0:
             raise ValueError("device must be a non-negative integer.")
+        if not (math.isfinite(self.tile_threshold) and 0.0 <= self.tile_threshold < 1.0):
+            raise ValueError("tile_threshold must lie in [0, 1).")
 
 
 @dataclass

@@ -50,7 +50,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_native.m1_table) == 32
     assert ctypes.sizeof(_native.m1_material) == 3 * 32 + 16
     assert ctypes.sizeof(_native.m1_beam) == 24
-    assert ctypes.sizeof(_native.m1_numerics) == 6 * 4 + 4 * 8
+    assert ctypes.sizeof(_native.m1_numerics) == 6 * 4 + 5 * 8
     assert _native.m1_numerics.max_energy_step.offset == 24
 
 

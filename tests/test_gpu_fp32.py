@@ -88,6 +88,30 @@ def _accuracy(cfg, label):
     return md, mx, gam, dose
 
 
+def test_fp32_tile_skipping_matches_full_domain():
+    """Skipping vacuum tiles (threshold 1e-10 of the peak inflow fluence)
+    against evaluating every tile, same FP32-bulk march."""
+    import dataclasses
+
+    cfg = CONFIG1
+    grid, mm, inflow, ctx, orc = setup(cfg)
+    num = _native.numerics_struct(cfg, F32)
+    num.tile_threshold = 1e-10
+    ctx.reset_stats()
+    E1, _ = ctx.march(num)
+    d1 = ctx.get_dose()
+    ev, full = ctx.tile_stats()
+    num.tile_threshold = 0.0
+    ctx.set_state(None)
+    E0, _ = ctx.march(num)
+    d0 = ctx.get_dose()
+    err = metrics.max_abs_difference_over_max(d1, d0)
+    say("tile skipping: evaluated", ev, "of", full, "tiles; max|dD|/maxD", err,
+        "steps", len(E1) - 1, len(E0) - 1)
+    assert full > 0 and ev < full
+    assert err < 1e-5
+
+
 def test_fp32_config1_march_accuracy():
     md, mx, gam, _ = _accuracy(CONFIG1, "config1 fp32")
     assert abs(md) <= MAX_DOSE_TOL
