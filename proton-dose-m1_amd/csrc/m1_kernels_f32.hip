@@ -17,474 +17,24 @@
 //   * inflow ghost extrapolation (coefficients up to 136) in double;
 //   * dose accumulated in a compensated float pair (TwoSum), read out as
 //     double(hi) + double(lo).
-#include "m1_device.h"
+#include "m1_f32_common.h"
 
 namespace m1 {
 namespace f32 {
 namespace {
 
-constexpr int TY = F32_TY;
-constexpr int TZ = F32_TZ;
-constexpr int TX = F32_TX;
-constexpr int NT = TY * TZ;
-constexpr int PY = TY + 6;
-constexpr int PZ = TZ + 6;
-constexpr int NHALO = 6 * TZ + 6 * TY;
-constexpr double REAL_EPS = 1.0e-10;  // weno_z.py:10, realizability.py:182
-
-__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// WENO-Z5 left state (weno_z.py:13-87) in float.  The ratios tau5/(beta+eps)
-// are capped at 1e18: beyond it the dominant weight is 1 to float precision
-// and the cap keeps alpha_k = d_k (1 + r_k^2) finite.
-__device__ __forceinline__ float weno_left(float v0, float v1, float v2, float v3, float v4,
-                                           float eps)
-{
-    const float p0 = v0 * (1.f / 3.f) - v1 * (7.f / 6.f) + v2 * (11.f / 6.f);
-    const float p1 = v2 * (5.f / 6.f) - v1 * (1.f / 6.f) + v3 * (1.f / 3.f);
-    const float p2 = v2 * (1.f / 3.f) + v3 * (5.f / 6.f) - v4 * (1.f / 6.f);
-    float a = v0 - 2.f * v1 + v2;
-    float b = v0 - 4.f * v1 + 3.f * v2;
-    const float b0 = (13.f / 12.f) * (a * a) + 0.25f * (b * b);
-    a = v1 - 2.f * v2 + v3;
-    b = v1 - v3;
-    const float b1 = (13.f / 12.f) * (a * a) + 0.25f * (b * b);
-    a = v2 - 2.f * v3 + v4;
-    b = 3.f * v2 - 4.f * v3 + v4;
-    const float b2 = (13.f / 12.f) * (a * a) + 0.25f * (b * b);
-    const float tau = fabsf(b2 - b0);
-    const float r0 = fminf(tau * rcpf(b0 + eps), 1e18f);
-    const float r1 = fminf(tau * rcpf(b1 + eps), 1e18f);
-    const float r2 = fminf(tau * rcpf(b2 + eps), 1e18f);
-    const float a0 = 0.1f * (1.f + r0 * r0);
-    const float a1 = 0.6f * (1.f + r1 * r1);
-    const float a2 = 0.3f * (1.f + r2 * r2);
-    return (a0 * p0 + a1 * p1 + a2 * p2) * rcpf(a0 + a1 + a2);
-}
-
-// Realizability projection in double (realizability.py:132-176).
-__device__ __forceinline__ void project(double& p, double& x, double& y, double& z, double& jn)
-{
-    if (p < 0.0)
-        p = 0.0;
-    jn = sqrt(x * x + y * y + z * z);
-    const double jmax = (1.0 - REAL_EPS) * p;
-    if (jn > jmax) {
-        const double s = jmax / jn;
-        x *= s;
-        y *= s;
-        z *= s;
-        jn = jmax;
-    }
-}
-
-__device__ __forceinline__ void project_f(float u[4])
-{
-    double p = u[0], x = u[1], y = u[2], z = u[3], jn;
-    project(p, x, y, z, jn);
-    u[0] = float(p);
-    u[1] = float(x);
-    u[2] = float(y);
-    u[3] = float(z);
-}
-
-struct Closure {
-    double jn, f, chi, n[3], iso, dir;
-};
-
-// Closure of a projected state with |J| known (flux.py:10-61).
-__device__ __forceinline__ Closure closure(double phi, const double J[3], double jn)
-{
-    Closure c;
-    c.jn = jn;
-    double f = phi > 0.0 ? jn / phi : 0.0;
-    f = fmin(fmax(f, 0.0), 1.0);
-    c.f = f;
-    c.chi = (5.0 - 2.0 * sqrt(4.0 - 3.0 * f * f)) * (1.0 / 3.0);
-    const double inv = jn > 0.0 ? 1.0 / jn : 0.0;
-    c.n[0] = J[0] * inv;
-    c.n[1] = J[1] * inv;
-    c.n[2] = J[2] * inv;
-    c.iso = 0.5 * (1.0 - c.chi);
-    c.dir = 0.5 * (3.0 * c.chi - 1.0);
-    return c;
-}
-
-// Extreme eigenvalues along AXIS (wave_speeds.py:37-103) with the scaled floor.
-template <int AXIS>
-__device__ __forceinline__ void eigen(double phi, const Closure& c, double eps, double& lmin,
-                                      double& lmax)
-{
-    const double iso = 0.57735026918962576;
-    if (c.jn < eps) {
-        lmin = -iso;
-        lmax = iso;
-        return;
-    }
-    double f, chi;
-    if (phi >= eps && c.f <= 1.0 - 1.0e-10) {
-        f = c.f;
-        chi = c.chi;
-    } else {
-        f = fmin(c.jn / fmax(phi, eps), 1.0 - 1.0e-10);
-        chi = (5.0 - 2.0 * sqrt(4.0 - 3.0 * f * f)) * (1.0 / 3.0);
-    }
-    const double mu = c.n[AXIS];
-    const double den = 5.0 - 3.0 * chi;
-    const double disc =
-        fmax(4.0 * f * f * mu * mu - den * (mu * mu * (3.0 * chi - 1.0) - 2.0 * (1.0 - chi)), 0.0);
-    const double root = sqrt(disc);
-    const double inv = 1.0 / den;
-    lmin = (2.0 * f * mu - root) * inv;
-    lmax = (2.0 * f * mu + root) * inv;
-}
-
-template <int AXIS>
-__device__ __forceinline__ void phys_flux(double phi, const double J[3], const Closure& c,
-                                          double F[4])
-{
-    F[0] = J[AXIS];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        F[1 + k] = phi * (c.dir * c.n[AXIS] * c.n[k] + (k == AXIS ? c.iso : 0.0));
-}
-
-// Face flux from six cell states s[c][0..5] (cells f-3..f+2 of face f-1/2).
-template <int AXIS>
-__device__ __forceinline__ void face_flux(const float s[4][6], int wave, float weps, double eeps,
-                                          float F[4])
-{
-    double L[4], R[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        L[c] = weno_left(s[c][0], s[c][1], s[c][2], s[c][3], s[c][4], weps);
-        R[c] = weno_left(s[c][5], s[c][4], s[c][3], s[c][2], s[c][1], weps);
-    }
-    double jl, jr;
-    project(L[0], L[1], L[2], L[3], jl);
-    project(R[0], R[1], R[2], R[3], jr);
-    const Closure cl = closure(L[0], L + 1, jl);
-    const Closure cr = closure(R[0], R + 1, jr);
-    double FL[4], FR[4];
-    phys_flux<AXIS>(L[0], L + 1, cl, FL);
-    phys_flux<AXIS>(R[0], R + 1, cr, FR);
-    double sl = -1.0, sr = 1.0;
-    if (wave == 0) {
-        double a, b, c, d;
-        eigen<AXIS>(L[0], cl, eeps, a, b);
-        eigen<AXIS>(R[0], cr, eeps, c, d);
-        sl = fmin(fmin(a, c), 0.0);
-        sr = fmax(fmax(b, d), 0.0);
-    }
-    const double inv = 1.0 / (sr - sl);
-    const double sp = sl * sr;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-        F[c] = float((sr * FL[c] - sl * FR[c] + sp * (R[c] - L[c])) * inv);
-}
-
-__device__ __forceinline__ void face_flux_yz(bool zaxis, const float s[4][6], int wave,
-                                             float weps, double eeps, float F[4])
-{
-    if (zaxis)
-        face_flux<2>(s, wave, weps, eeps, F);
-    else
-        face_flux<1>(s, wave, weps, eeps, F);
-}
-
-// Stencil input of one cell: U directly, or proj(C_k V_k / S_k) for stages 1, 2.
-template <int KIND>
-__device__ __forceinline__ void load_state(const StageArgs32& a, const StepCoef32& c, long i,
-                                           long N, float u[4])
-{
-    u[0] = a.src[i];
-    u[1] = a.src[i + N];
-    u[2] = a.src[i + 2 * N];
-    u[3] = a.src[i + 3 * N];
-    if constexpr (KIND == STAGE1 || KIND == STAGE2) {
-        const int m = a.ids[i];
-        const float gp = KIND == STAGE1 ? c.g1p[m] : c.g2p[m];
-        const float gj = KIND == STAGE1 ? c.g1j[m] : c.g2j[m];
-        u[0] *= gp;
-        u[1] *= gj;
-        u[2] *= gj;
-        u[3] *= gj;
-        project_f(u);
-    }
-}
-
-// Quartic z-min ghost k (boundary.py:274-299) in double.
-__device__ __forceinline__ float zmin_ghost(int k, double uin, double u0, double u1, double u2,
-                                            double u3)
-{
-    double g;
-    if (k == 1)
-        g = 5.0 * uin - (77.0 / 12.0) * u0 + (43.0 / 12.0) * u1 - (17.0 / 12.0) * u2 + 0.25 * u3;
-    else if (k == 2)
-        g = 25.0 * uin - (505.0 / 12.0) * u0 + (335.0 / 12.0) * u1 - (145.0 / 12.0) * u2 +
-            (9.0 / 4.0) * u3;
-    else
-        g = 75.0 * uin - (545.0 / 4.0) * u0 + (395.0 / 4.0) * u1 - (181.0 / 4.0) * u2 +
-            (35.0 / 4.0) * u3;
-    return float(g);
-}
-
-// Compensated accumulation hi + lo += v (Knuth TwoSum).
-__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
-{
-    const float s = hi + v;
-    const float bp = s - hi;
-    const float err = (hi - (s - bp)) + (v - bp);
-    hi = s;
-    lo += err;
-}
+using namespace detail;
 
 template <int KIND, bool IFAC>
 __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
 {
-    __shared__ StepCoef32 sc;
-    __shared__ float P[4][PY][PZ];
-    __shared__ float Fy[4][TY + 1][TZ];
-    __shared__ float Fz[4][TY][TZ + 1];
-    __shared__ float wmax[NT / 64];
-
     const long tile = (long(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     if (a.compute && !a.compute[tile])
         return;  // whole tile and its 9-cell neighbourhood are vacuum
-    const int tid = threadIdx.x;
-    float phimax = 0.f;
-    {
-        const unsigned* src = reinterpret_cast<const unsigned*>(a.coef);
-        unsigned* dst = reinterpret_cast<unsigned*>(&sc);
-        for (int k = tid; k < int(sizeof(StepCoef32) / 4); k += NT)
-            dst[k] = src[k];
-    }
-    __syncthreads();
-
-    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-    const long N = long(nx) * ny * nz;
-    const int tz = tid % TZ, ty = tid / TZ;
-    const int z0 = blockIdx.x * TZ, y0 = blockIdx.y * TY, x0 = blockIdx.z * TX;
-    const int z = z0 + tz, y = y0 + ty;
-    const int zc = min(z, nz - 1), yc = min(y, ny - 1);
-    const bool valid = z < nz && y < ny;
-    const int x1 = min(x0 + TX, nx);
-    const int wave = a.wave_mode;
-    const float weps = sc.weno_eps;
-    const double eeps = sc.eig_eps;
-    const float idx = float(1.0 / a.g.dx), idy = float(1.0 / a.g.dy), idz = float(1.0 / a.g.dz);
-
-    auto cidx = [&](int xx, int yy, int zz) -> long { return (long(xx) * ny + yy) * nz + zz; };
-    auto clampx = [&](int xx) { return xx < 0 ? 0 : (xx >= nx ? nx - 1 : xx); };
-
-    int hy = -1, hz = -1;
-    if (tid < 6 * TZ) {
-        const int r = tid / TZ, col = tid % TZ;
-        hy = r < 3 ? r : TY + r;
-        hz = col + 3;
-    } else if (tid < NHALO) {
-        const int h = tid - 6 * TZ;
-        const int r = h / 6, c6 = h % 6;
-        hy = r + 3;
-        hz = c6 < 3 ? c6 : TZ + c6;
-    }
-    const int hgy = min(max(y0 + hy - 3, 0), ny - 1);
-    const int hgz_raw = z0 + hz - 3;
-    const int hgz = min(max(hgz_raw, 0), nz - 1);
-
-    float W[6][4];
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        load_state<KIND>(a, sc, cidx(clampx(x0 - 3 + k), yc, zc), N, W[k]);
-    float Flo[4];
-    {
-        float s[4][6];
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-                s[c][k] = W[k][c];
-        face_flux<0>(s, wave, weps, eeps, Flo);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            W[k][c] = W[k + 1][c];
-
-    for (int x = x0; x < x1; ++x) {
-        load_state<KIND>(a, sc, cidx(clampx(x + 3), yc, zc), N, W[5]);
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            P[c][ty + 3][tz + 3] = W[2][c];
-        if (hy >= 0) {
-            float u[4];
-            if (hgz_raw < 0) {
-                float q[4][4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    load_state<KIND>(a, sc, cidx(x, hgy, k), N, q[k]);
-                const long b = long(x) * ny + hgy;
-                const long m2 = long(nx) * ny;
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    u[c] = zmin_ghost(-hgz_raw, a.bnd[c * m2 + b], q[0][c], q[1][c], q[2][c],
-                                      q[3][c]);
-            } else {
-                load_state<KIND>(a, sc, cidx(x, hgy, hgz), N, u);
-            }
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                P[c][hy][hz] = u[c];
-        }
-        float Fhi[4];
-        {
-            float s[4][6];
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = W[k][c];
-            face_flux<0>(s, wave, weps, eeps, Fhi);
-        }
-        __syncthreads();
-        {
-            float s[4][6], F[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = P[c][ty + 3][tz + k];
-            face_flux<2>(s, wave, weps, eeps, F);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                Fz[c][ty][tz] = F[c];
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = P[c][ty + k][tz + 3];
-            face_flux<1>(s, wave, weps, eeps, F);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                Fy[c][ty][tz] = F[c];
-        }
-        if (tid < TY + TZ) {
-            const bool zaxis = tid < TY;
-            const int row = zaxis ? tid : 0;
-            const int col = zaxis ? 0 : tid - TY;
-            float s[4][6], F[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = zaxis ? P[c][row + 3][TZ + k] : P[c][TY + k][col + 3];
-            face_flux_yz(zaxis, s, wave, weps, eeps, F);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if (zaxis)
-                    Fz[c][row][TZ] = F[c];
-                else
-                    Fy[c][TY][col] = F[c];
-            }
-        }
-        __syncthreads();
-
-        if (valid) {
-            const long i = cidx(x, y, z);
-            const int m = a.ids[i];
-            const float dE = sc.dE;
-            float rhs[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                rhs[c] = -((Fhi[c] - Flo[c]) * idx + (Fy[c][ty + 1][tz] - Fy[c][ty][tz]) * idy +
-                           (Fz[c][ty][tz + 1] - Fz[c][ty][tz]) * idz);
-            if constexpr (KIND == STAGE_RHS) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = rhs[c];
-            } else if constexpr (KIND == STAGE0) {
-                const float S0 = sc.S0[m];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = rhs[c] * dE + S0 * W[2][c];
-            } else if constexpr (KIND == STAGE_FE) {
-                const float S0 = sc.S0[m];
-                float u[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    u[c] = (rhs[c] * dE + S0 * W[2][c]) * (1.f / sc.S1[m]);
-                project_f(u);
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = u[c];
-                const float e = dE * W[2][0] * S0;
-                if (a.edep)
-                    a.edep[i] = e;
-                if (a.dose_hi)
-                    two_sum_add(a.dose_hi[i], a.dose_lo[i], e * sc.dose_scale[m]);
-            } else if constexpr (KIND == STAGE1) {
-                const float S0 = sc.S0[m];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float r = rhs[c];
-                    if constexpr (IFAC)
-                        r *= (c == 0 ? sc.i1p[m] : sc.i1j[m]);
-                    a.out[i + c * N] = 0.75f * (S0 * a.U[i + c * N]) + 0.25f * (r * dE + a.V1[i + c * N]);
-                }
-            } else {  // STAGE2
-                const float S0 = sc.S0[m], S2 = sc.S2[m];
-                float un[4], u[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    un[c] = a.U[i + c * N];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float r = rhs[c];
-                    if constexpr (IFAC)
-                        r *= (c == 0 ? sc.i2p[m] : sc.i2j[m]);
-                    const float v = (1.f / 3.f) * (S0 * un[c]) + (2.f / 3.f) * (r * dE + a.src[i + c * N]);
-                    u[c] = v * (c == 0 ? sc.g1p[m] : sc.g1j[m]);
-                }
-                project_f(u);
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.U[i + c * N] = u[c];
-                phimax = fmaxf(phimax, u[0]);
-                const float q0 = un[0] * S0;
-                const float q1 = fmaxf(a.V1[i] * sc.g1p[m], 0.f) * sc.S1[m];
-                const float q2 = W[2][0] * S2;
-                const float e = dE * ((1.f / 6.f) * (q0 + q1) + (2.f / 3.f) * q2);
-                if (a.edep)
-                    a.edep[i] = e;
-                if (a.dose_hi)
-                    two_sum_add(a.dose_hi[i], a.dose_lo[i], e * sc.dose_scale[m]);
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            Flo[c] = Fhi[c];
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                W[k][c] = W[k + 1][c];
-    }
-    if constexpr (KIND == STAGE2) {
-        if (a.tile_max) {
-            for (int off = 32; off > 0; off >>= 1)
-                phimax = fmaxf(phimax, __shfl_xor(phimax, off, 64));
-            if ((tid & 63) == 0)
-                wmax[tid >> 6] = phimax;
-            __syncthreads();
-            if (tid == 0) {
-                float m = wmax[0];
-                for (int w = 1; w < NT / 64; ++w)
-                    m = fmaxf(m, wmax[w]);
-                a.tile_max[tile] = m;
-            }
-        }
-    }
+    TileView v{a.g,      a.src,       a.U,   a.V1,  a.out,  a.dose_hi,
+               a.dose_lo, a.edep,     a.ids, a.bnd, a.coef, a.tile_max ? a.tile_max + tile : nullptr,
+               a.wave_mode, int(blockIdx.z), int(blockIdx.y), int(blockIdx.x)};
+    stage_tile<KIND, IFAC>(v);
 }
 
 // ---------------------------------------------------------------- CFL
