@@ -15,43 +15,35 @@
 #include <string>
 #include <vector>
 
-#include "m1_device.h"
-#include "m1_physics.h"
-#include "m1march.h"
+#include "m1_host.h"
 
+using m1::BeamParams;
+using m1::HipError;
 using m1::MaterialData;
+using m1::NonFinite;
+using m1::NotImplemented;
+using m1::Num;
 using m1::StepCoef;
+using m1::Timed;
 using m1::ValueError;
+using m1::dev_alloc;
+using m1::has_sources;
+using m1::hip_check;
+using m1::read_numerics;
+using m1::K_STAGE0;
+using m1::K_STAGE1;
+using m1::K_STAGE2;
+using m1::K_RHS;
+using m1::K_FE;
+using m1::K_CFL;
+using m1::K_BOUNDARY;
+using m1::K_SCALE;
+using m1::K_TILE_MASK;
+using m1::K_TILE_ZERO;
+using m1::K_COUNT;
 
 namespace {
-
-enum KernelId {
-    K_STAGE0, K_STAGE1, K_STAGE2, K_RHS, K_FE, K_CFL, K_BOUNDARY, K_SCALE, K_COUNT
-};
-const char* const kKernelNames[K_COUNT] = {
-    "stage0", "stage1", "stage2", "spatial_operator", "forward_euler", "cfl", "boundary",
-    "source_half_step"};
-
 std::string g_create_error;
-
-struct HipError {
-    std::string msg;
-};
-
-void hip_check(hipError_t e, const char* what)
-{
-    if (e != hipSuccess)
-        throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-struct NotImplemented {
-    std::string msg;
-};
-
-struct NonFinite {
-    std::string msg;
-};
-
 }  // namespace
 
 struct m1_ctx {
@@ -92,15 +84,7 @@ struct m1_ctx {
     int pending_tiles = 0;
     unsigned long long* h_red = nullptr;
 
-    bool prof = false;
-    struct Pending {
-        int kernel;
-        hipEvent_t a, b;
-    };
-    std::vector<hipEvent_t> pool;
-    std::vector<Pending> pending;
-    int64_t launches[K_COUNT] = {};
-    double total_ms[K_COUNT] = {};
+    m1::Profiler prof;
 
     void free_problem();
     ~m1_ctx();
@@ -136,193 +120,33 @@ m1_ctx::~m1_ctx()
     for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red, (void*)h_step32, (void*)h_nactive})
         if (p)
             (void)hipHostFree(p);
-    for (auto& e : pool)
-        (void)hipEventDestroy(e);
-    for (auto& p : pending) {
-        (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
-    }
+    prof.release();
     if (stream)
         (void)hipStreamDestroy(stream);
 }
 
 namespace {
 
-// ------------------------------------------------------------- profiling
-hipEvent_t take_event(m1_ctx* c)
-{
-    if (!c->pool.empty()) {
-        hipEvent_t e = c->pool.back();
-        c->pool.pop_back();
-        return e;
-    }
-    hipEvent_t e;
-    hip_check(hipEventCreate(&e), "hipEventCreate");
-    return e;
-}
-
-struct Timed {
-    m1_ctx* c;
-    int k;
-    hipEvent_t a = nullptr, b = nullptr;
-    Timed(m1_ctx* ctx, int kernel) : c(ctx), k(kernel)
-    {
-        if (c->prof) {
-            a = take_event(c);
-            b = take_event(c);
-            hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
-        }
-    }
-    ~Timed()
-    {
-        if (c->prof) {
-            (void)hipEventRecord(b, c->stream);
-            c->pending.push_back({k, a, b});
-        }
-    }
-};
-
-// Fold finished event pairs into the statistics (after a stream sync).
-void collect(m1_ctx* c)
-{
-    for (auto& p : c->pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-            c->launches[p.kernel] += 1;
-            c->total_ms[p.kernel] += ms;
-        }
-        c->pool.push_back(p.a);
-        c->pool.push_back(p.b);
-    }
-    c->pending.clear();
-}
-
 void sync(m1_ctx* c)
 {
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    if (c->prof)
-        collect(c);
+    if (c->prof.on)
+        c->prof.collect();
 }
 
 // ------------------------------------------------------------ numerics
-struct Num {
-    int scheme, wave, scattering, nuclear, interp, precision;
-    double max_step, cfl, emin, window, tile_threshold;
-};
-
-Num read_numerics(const m1_numerics* n)
-{
-    if (!n)
-        throw ValueError{"numerics must not be NULL"};
-    Num r{n->energy_scheme, n->wave_speeds, n->scattering, n->nuclear,
-          n->coefficient_interpolation, n->precision, n->max_energy_step, n->cfl,
-          n->energy_min, n->energy_window_sigma, n->tile_threshold};
-    if (r.scheme < M1_FORWARD_EULER || r.scheme > M1_IF_SSPRK3)
-        throw NotImplemented{"Energy scheme has not been migrated yet."};
-    if (r.wave != M1_WAVE_EXACT && r.wave != M1_WAVE_UNIT)
-        throw ValueError{"Unknown wave-speed mode."};
-    if (r.scattering < M1_SCATTERING_OFF || r.scattering > M1_SCATTERING_WENTZEL)
-        throw ValueError{"Unknown scattering model."};
-    if (r.interp != M1_INTERP_LINEAR && r.interp != M1_INTERP_PCHIP)
-        throw ValueError{"Unknown coefficient interpolation."};
-    if (r.precision != M1_FP64 && r.precision != M1_FP64_FAST && r.precision != M1_FP32_BULK)
-        throw ValueError{"Unknown precision."};
-    if (!(std::isfinite(r.max_step) && std::isfinite(r.cfl) && std::isfinite(r.emin) &&
-          std::isfinite(r.window)))
-        throw ValueError{"Numerical parameters must be finite."};
-    if (r.max_step <= 0)
-        throw ValueError{"max_energy_step must be positive."};
-    if (!(r.cfl > 0 && r.cfl <= 1))
-        throw ValueError{"cfl must lie in (0, 1]."};
-    if (r.emin < 0)
-        throw ValueError{"energy_min cannot be negative."};
-    if (!(r.tile_threshold >= 0.0 && r.tile_threshold < 1.0))
-        throw ValueError{"tile_threshold must lie in [0, 1)."};
-    return r;
-}
-
-bool has_sources(const Num& n) { return n.scattering != M1_SCATTERING_OFF || n.nuclear; }
-
 m1::GridDims dims(const m1_ctx* c) { return {c->nx, c->ny, c->nz, c->dx, c->dy, c->dz}; }
 
-// Per-material optical depths over [E - d, E] (energy.py:234-270).
-void depths(const m1_ctx* c, const Num& n, double energy, double d, double* ts, double* tn)
+BeamParams beam_params(const m1_ctx* c)
 {
-    for (size_t m = 0; m < c->mats.size(); ++m) {
-        ts[m] = 0.0;
-        tn[m] = 0.0;
-        if (d <= 0.0)
-            continue;
-        if (n.scattering != M1_SCATTERING_OFF)
-            ts[m] = m1::scattering_depth(c->mats[m], n.scattering, n.interp, energy, d);
-        if (n.nuclear)
-            tn[m] = m1::nuclear_depth(c->mats[m], n.interp, energy, d);
-    }
-}
-
-// exp(+-min(tau_n, 700)), exp(+-min(tau_n + tau_s, 700)) (energy.py:294-308).
-void factors(size_t nm, const double* ts, const double* tn, bool inverse, double* fp, double* fj)
-{
-    const double sign = inverse ? 1.0 : -1.0;
-    for (size_t m = 0; m < nm; ++m) {
-        const double a = std::min(tn[m], 700.0);
-        const double b = std::min(tn[m] + ts[m], 700.0);
-        fp[m] = std::exp(sign * a);
-        fj[m] = std::exp(sign * b);
-    }
+    return {c->beam_energy, c->energy_spread, c->angular_spread};
 }
 
 // All scalars of one step at (energy, dE).
 void fill_step(const m1_ctx* c, const Num& n, double energy, double dE, bool ifac, bool strang,
                StepCoef& s)
 {
-    std::memset(&s, 0, sizeof s);
-    const size_t nm = c->mats.size();
-    s.dE = dE;
-    const double e13 = energy - dE / 3.0, e23 = energy - 2.0 * dE / 3.0, e1 = energy - dE,
-                 e2 = energy - 0.5 * dE;
-    s.ed[0] = m1::energy_density(energy, c->beam_energy, c->energy_spread);
-    s.ed[1] = m1::energy_density(e13, c->beam_energy, c->energy_spread);
-    s.ed[2] = m1::energy_density(e23, c->beam_energy, c->energy_spread);
-    s.ed[3] = m1::energy_density(e1, c->beam_energy, c->energy_spread);
-    s.rf = std::exp(-(c->angular_spread * c->angular_spread));
-    for (size_t m = 0; m < nm; ++m) {
-        const MaterialData& md = c->mats[m];
-        s.S0[m] = m1::stopping_power(md, energy);
-        s.S1[m] = m1::stopping_power(md, e1);
-        s.S2[m] = m1::stopping_power(md, e2);
-        s.S13[m] = m1::stopping_power(md, e13);
-        s.S23[m] = m1::stopping_power(md, e23);
-        s.rho[m] = md.density;
-        s.f1p[m] = s.f1j[m] = s.f2p[m] = s.f2j[m] = 1.0;
-        s.i1p[m] = s.i1j[m] = s.i2p[m] = s.i2j[m] = 1.0;
-        s.i13p[m] = s.i13j[m] = s.i23p[m] = s.i23j[m] = 1.0;
-        s.hn[m] = s.hs[m] = 1.0;
-    }
-    if (ifac) {
-        double ts[4][m1::MAXMAT], tn[4][m1::MAXMAT];
-        const double ds[4] = {dE / 3.0, 0.5 * dE, 2.0 * dE / 3.0, dE};
-        for (int q = 0; q < 4; ++q)
-            depths(c, n, energy, ds[q], ts[q], tn[q]);
-        factors(nm, ts[3], tn[3], false, s.f1p, s.f1j);
-        factors(nm, ts[1], tn[1], false, s.f2p, s.f2j);
-        factors(nm, ts[3], tn[3], true, s.i1p, s.i1j);
-        factors(nm, ts[1], tn[1], true, s.i2p, s.i2j);
-        factors(nm, ts[0], tn[0], true, s.i13p, s.i13j);
-        factors(nm, ts[2], tn[2], true, s.i23p, s.i23j);
-    }
-    if (strang) {
-        // energy.py:1206-1227 with the half-step factors of 1168-1191.
-        for (size_t m = 0; m < nm; ++m) {
-            double eta = 0.0, taun = 0.0;
-            if (n.scattering != M1_SCATTERING_OFF)
-                eta = m1::scattering_depth(c->mats[m], n.scattering, n.interp, energy, dE);
-            if (n.nuclear)
-                taun = m1::nuclear_depth(c->mats[m], n.interp, energy, dE);
-            s.hn[m] = std::exp(-std::min(0.5 * taun, 700.0));
-            s.hs[m] = std::exp(-std::min(0.5 * eta, 700.0));
-        }
-    }
+    m1::fill_step(c->mats, beam_params(c), n, energy, dE, ifac, strang, s);
 }
 
 hipError_t stage(const m1_ctx* c, const Num& n, int kind, bool ifac, const m1::StageArgs& a)
@@ -352,7 +176,7 @@ void upload_step(m1_ctx* c)
 void run_boundary(m1_ctx* c, const Num& n, bool ifac)
 {
     m1::BoundaryArgs b{dims(c), c->spatial, c->ids, c->d_step, c->bnd, ifac ? 1 : 0};
-    Timed t(c, K_BOUNDARY);
+    Timed t(c->prof, c->stream, K_BOUNDARY);
     hipError_t e = n.precision == M1_FP64 ? m1::exact::launch_boundary(b, c->stream)
                                                  : m1::fast::launch_boundary(b, c->stream);
     hip_check(e, "boundary kernel");
@@ -362,7 +186,7 @@ void run_scale(m1_ctx* c, const Num& n)
 {
     m1::ScaleArgs s{c->N, c->U, c->ids, c->d_step, n.nuclear ? 1 : 0,
                     n.scattering != M1_SCATTERING_OFF ? 1 : 0};
-    Timed t(c, K_SCALE);
+    Timed t(c->prof, c->stream, K_SCALE);
     hip_check(m1::fast::launch_scale(s, c->stream), "source half-step kernel");
 }
 
@@ -379,14 +203,14 @@ void rk3(m1_ctx* c, const Num& n, bool ifac, double* dose, double* edep)
         a.src = c->U;
         a.out = c->V1;
         a.bnd = c->bnd;
-        Timed t(c, K_STAGE0);
+        Timed t(c->prof, c->stream, K_STAGE0);
         hip_check(stage(c, n, m1::STAGE0, false, a), "stage 0 kernel");
     }
     {
         a.src = c->V1;
         a.out = c->V2;
         a.bnd = c->bnd + 4 * m2;
-        Timed t(c, K_STAGE1);
+        Timed t(c->prof, c->stream, K_STAGE1);
         hip_check(stage(c, n, m1::STAGE1, ifac, a), "stage 1 kernel");
     }
     {
@@ -395,7 +219,7 @@ void rk3(m1_ctx* c, const Num& n, bool ifac, double* dose, double* edep)
         a.dose = dose;
         a.edep = edep;
         a.bnd = c->bnd + 8 * m2;
-        Timed t(c, K_STAGE2);
+        Timed t(c->prof, c->stream, K_STAGE2);
         hip_check(stage(c, n, m1::STAGE2, ifac, a), "stage 2 kernel");
     }
 }
@@ -412,7 +236,7 @@ void forward_euler(m1_ctx* c, const Num& n, double* dose, double* edep)
     a.edep = edep;
     a.bnd = c->bnd;
     {
-        Timed t(c, K_FE);
+        Timed t(c->prof, c->stream, K_FE);
         hip_check(stage(c, n, m1::STAGE_FE, false, a), "forward Euler kernel");
     }
     std::swap(c->U, c->work);
@@ -463,11 +287,7 @@ void step(m1_ctx* c, const Num& n, int kind, double energy, double dE, double* d
 double cfl_step(m1_ctx* c, const Num& n, double energy)
 {
     StepCoef& s = *c->h_cfl;
-    std::memset(&s, 0, sizeof s);
-    for (size_t m = 0; m < c->mats.size(); ++m)
-        s.S0[m] = m1::stopping_power(c->mats[m], energy);
-    s.ed[0] = m1::energy_density(energy, c->beam_energy, c->energy_spread);
-    s.rf = std::exp(-(c->angular_spread * c->angular_spread));
+    m1::fill_cfl(c->mats, beam_params(c), energy, s);
     hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, sizeof(StepCoef), hipMemcpyHostToDevice,
                              c->stream),
               "CFL coefficient upload");
@@ -475,7 +295,7 @@ double cfl_step(m1_ctx* c, const Num& n, double energy)
               "CFL reset");
     m1::CflArgs a{dims(c), c->U, c->ids, c->spatial, c->d_cfl, c->d_red, n.wave};
     {
-        Timed t(c, K_CFL);
+        Timed t(c->prof, c->stream, K_CFL);
         hipError_t e = n.precision == M1_FP64 ? m1::exact::launch_cfl(a, c->stream)
                                                      : m1::fast::launch_cfl(a, c->stream);
         hip_check(e, "CFL kernel");
@@ -486,10 +306,7 @@ double cfl_step(m1_ctx* c, const Num& n, double energy)
     sync(c);
     double mx;
     std::memcpy(&mx, c->h_red, sizeof mx);
-    if (!std::isfinite(mx))
-        throw NonFinite{"non-finite M1 state detected by the CFL reduction"};
-    const double dE = n.cfl / mx;
-    return dE <= n.max_step ? dE : n.max_step;
+    return m1::cfl_energy_step(n, mx);
 }
 
 void require_problem(const m1_ctx* c)
@@ -498,71 +315,14 @@ void require_problem(const m1_ctx* c)
         throw ValueError{"no problem bound: call m1_set_problem first"};
 }
 
-MaterialData read_material(const m1_material& m, int idx)
-{
-    auto table = [&](const m1_table& t, bool need_slope, const char* what) {
-        m1::Table r;
-        if (t.n < 2 || !t.energy || !t.value)
-            throw ValueError{std::string(what) + " table of material " + std::to_string(idx) +
-                             " must contain at least two points."};
-        r.E.assign(t.energy, t.energy + t.n);
-        r.v.assign(t.value, t.value + t.n);
-        for (int i = 1; i < t.n; ++i)
-            if (!(r.E[i] > r.E[i - 1]))
-                throw ValueError{std::string(what) + " energies must be strictly increasing."};
-        if (t.slope)
-            r.slope.assign(t.slope, t.slope + t.n);
-        else if (need_slope)
-            throw ValueError{std::string(what) + " table needs PCHIP slopes."};
-        return r;
-    };
-    MaterialData d;
-    d.sp = table(m.stopping_power, true, "Stopping-power");
-    for (double v : d.sp.v)
-        if (!(std::isfinite(v) && v > 0.0))
-            throw ValueError{"Stopping-power values must be finite and positive."};
-    d.tr = table(m.transport, false, "Transport");
-    d.nu = table(m.removal, false, "Nuclear-attenuation");
-    d.density = m.density;
-    d.radiation_length = m.radiation_length;
-    if (!(d.density > 0.0 && std::isfinite(d.density)))
-        throw ValueError{"material density must be finite and positive"};
-    return d;
-}
-
 template <class F>
 int guarded(m1_ctx* c, F&& f)
 {
     if (!c)
         return M1_ERR_STATE;
-    try {
-        hip_check(hipSetDevice(c->device), "hipSetDevice");
-        f();
-        c->err.clear();
-        return M1_OK;
-    } catch (const ValueError& e) {
-        c->err = e.msg;
-        return M1_ERR_VALUE;
-    } catch (const NotImplemented& e) {
-        c->err = e.msg;
-        return M1_ERR_NOT_IMPLEMENTED;
-    } catch (const NonFinite& e) {
-        c->err = e.msg;
-        return M1_ERR_NONFINITE;
-    } catch (const HipError& e) {
-        c->err = e.msg;
-        return M1_ERR_HIP;
-    } catch (const std::exception& e) {
-        c->err = e.what();
-        return M1_ERR_HIP;
-    }
+    return m1::run_guarded(c->device, c->err, f);
 }
 
-template <class T>
-void dev_alloc(T** p, size_t count, const char* what)
-{
-    hip_check(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)), what);
-}
 
 // ------------------------------------------------------------- FP32 bulk
 // The FP32-bulk march keeps the device state as U / alpha in float, with
@@ -596,51 +356,10 @@ void ensure_f32(m1_ctx* c)
     }
 }
 
-double march_alpha(const m1_ctx* c, const std::vector<double>& profile_max)
-{
-    const double pk = profile_max.empty() ? 0.0 : profile_max[0];
-    const double ed = m1::energy_density(c->beam_energy, c->beam_energy, c->energy_spread);
-    const double a = pk * ed;
-    return a > 0.0 && std::isfinite(a) ? a : 1.0;
-}
-
-void set_norm(const m1_ctx* c, StepCoef& s)
-{
-    s.alpha = c->alpha;
-    s.inv_alpha = 1.0 / c->alpha;
-    s.eig_eps = 1.0e-10 / c->alpha;
-    s.weno_eps = 1.0e-12 / (c->alpha * c->alpha);
-}
-
-// Float stage scalars from the double ones.
-void fill_step32(const m1_ctx* c, const StepCoef& d, m1::StepCoef32& f)
-{
-    std::memset(&f, 0, sizeof f);
-    f.dE = float(d.dE);
-    f.weno_eps = float(std::max(d.weno_eps, 1.0e-37));
-    f.eig_eps = d.eig_eps;
-    for (size_t m = 0; m < c->mats.size(); ++m) {
-        f.S0[m] = float(d.S0[m]);
-        f.S1[m] = float(d.S1[m]);
-        f.S2[m] = float(d.S2[m]);
-        f.g1p[m] = float(d.f1p[m] / d.S1[m]);
-        f.g1j[m] = float(d.f1j[m] / d.S1[m]);
-        f.g2p[m] = float(d.f2p[m] / d.S2[m]);
-        f.g2j[m] = float(d.f2j[m] / d.S2[m]);
-        f.i1p[m] = float(d.i1p[m]);
-        f.i1j[m] = float(d.i1j[m]);
-        f.i2p[m] = float(d.i2p[m]);
-        f.i2j[m] = float(d.i2j[m]);
-        f.n1p[m] = f.g1p[m];
-        f.n1j[m] = f.g1j[m];
-        f.dose_scale[m] = float(c->alpha * m1::MEV_TO_JOULE / d.rho[m]);
-    }
-}
-
 void upload_step32(m1_ctx* c)
 {
-    set_norm(c, *c->h_step);
-    fill_step32(c, *c->h_step, *c->h_step32);
+    m1::set_norm(c->alpha, *c->h_step);
+    m1::fill_step32(c->mats.size(), c->alpha, *c->h_step, *c->h_step32);
     hip_check(hipMemcpyAsync(c->d_step, c->h_step, sizeof(StepCoef), hipMemcpyHostToDevice,
                              c->stream),
               "coefficient upload");
@@ -671,7 +390,7 @@ void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
 {
     {
         m1::BoundaryArgs32 b{dims(c), c->spatial, c->ids, c->d_step, c->bnd32, ifac ? 1 : 0};
-        Timed t(c, K_BOUNDARY);
+        Timed t(c->prof, c->stream, K_BOUNDARY);
         hip_check(m1::f32::launch_boundary(b, c->stream), "boundary kernel (f32)");
     }
     const long m2 = long(c->nx) * c->ny;
@@ -684,14 +403,14 @@ void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
         a.src = c->U32;
         a.out = c->V1_32;
         a.bnd = c->bnd32;
-        Timed t(c, K_STAGE0);
+        Timed t(c->prof, c->stream, K_STAGE0);
         hip_check(m1::f32::launch_stage(m1::STAGE0, false, a, c->stream), "stage 0 kernel (f32)");
     }
     {
         a.src = c->V1_32;
         a.out = c->V2_32;
         a.bnd = c->bnd32 + 4 * m2;
-        Timed t(c, K_STAGE1);
+        Timed t(c->prof, c->stream, K_STAGE1);
         hip_check(m1::f32::launch_stage(m1::STAGE1, ifac, a, c->stream), "stage 1 kernel (f32)");
     }
     {
@@ -701,7 +420,7 @@ void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
         a.dose_lo = dose ? c->dose_lo : nullptr;
         a.edep = edep ? c->edep32 : nullptr;
         a.bnd = c->bnd32 + 8 * m2;
-        Timed t(c, K_STAGE2);
+        Timed t(c->prof, c->stream, K_STAGE2);
         hip_check(m1::f32::launch_stage(m1::STAGE2, ifac, a, c->stream), "stage 2 kernel (f32)");
     }
 }
@@ -710,7 +429,7 @@ void forward_euler_f32(m1_ctx* c, const Num& n, bool dose, bool edep)
 {
     {
         m1::BoundaryArgs32 b{dims(c), c->spatial, c->ids, c->d_step, c->bnd32, 0};
-        Timed t(c, K_BOUNDARY);
+        Timed t(c->prof, c->stream, K_BOUNDARY);
         hip_check(m1::f32::launch_boundary(b, c->stream), "boundary kernel (f32)");
     }
     m1::StageArgs32 a = base_args32(c, n);
@@ -722,7 +441,7 @@ void forward_euler_f32(m1_ctx* c, const Num& n, bool dose, bool edep)
     a.edep = edep ? c->edep32 : nullptr;
     a.bnd = c->bnd32;
     {
-        Timed t(c, K_FE);
+        Timed t(c->prof, c->stream, K_FE);
         hip_check(m1::f32::launch_stage(m1::STAGE_FE, false, a, c->stream),
                   "forward Euler kernel (f32)");
     }
@@ -731,7 +450,7 @@ void forward_euler_f32(m1_ctx* c, const Num& n, bool dose, bool edep)
 
 void run_scale_f32(m1_ctx* c, const Num& n)
 {
-    Timed t(c, K_SCALE);
+    Timed t(c->prof, c->stream, K_SCALE);
     hip_check(m1::f32::launch_scale(c->N, c->U32, c->ids, c->d_step, n.nuclear ? 1 : 0,
                                     n.scattering != M1_SCATTERING_OFF ? 1 : 0, c->stream),
               "source half-step kernel (f32)");
@@ -776,19 +495,15 @@ void step_f32(m1_ctx* c, const Num& n, int kind, double energy, double dE, bool 
 double cfl_step_f32(m1_ctx* c, const Num& n, double energy)
 {
     StepCoef& s = *c->h_cfl;
-    std::memset(&s, 0, sizeof s);
-    for (size_t m = 0; m < c->mats.size(); ++m)
-        s.S0[m] = m1::stopping_power(c->mats[m], energy);
-    s.ed[0] = m1::energy_density(energy, c->beam_energy, c->energy_spread);
-    s.rf = std::exp(-(c->angular_spread * c->angular_spread));
-    set_norm(c, s);
+    m1::fill_cfl(c->mats, beam_params(c), energy, s);
+    m1::set_norm(c->alpha, s);
     hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, sizeof(StepCoef), hipMemcpyHostToDevice,
                              c->stream),
               "CFL coefficient upload");
     hip_check(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream), "CFL reset");
     m1::CflArgs32 a{dims(c), c->U32, c->ids, c->spatial, c->d_cfl, c->d_red, n.wave};
     {
-        Timed t(c, K_CFL);
+        Timed t(c->prof, c->stream, K_CFL);
         hip_check(m1::f32::launch_cfl(a, c->stream), "CFL kernel (f32)");
     }
     hip_check(hipMemcpyAsync(c->h_red, c->d_red, sizeof(unsigned long long),
@@ -801,10 +516,7 @@ double cfl_step_f32(m1_ctx* c, const Num& n, double energy)
     }
     double mx;
     std::memcpy(&mx, c->h_red, sizeof mx);
-    if (!std::isfinite(mx))
-        throw NonFinite{"non-finite M1 state detected by the CFL reduction"};
-    const double dE = n.cfl / mx;
-    return dE <= n.max_step ? dE : n.max_step;
+    return m1::cfl_energy_step(n, mx);
 }
 
 // Next step's tile set from this step's tile maxima (rk3 steps only; the
@@ -829,10 +541,16 @@ void update_tiles(m1_ctx* c, const Num& n, double energy_next)
     m1::TileMaskArgs ma{c->ntx, c->nty, c->ntz, c->d_tile_max, c->d_compute[c->cur],
                         c->d_compute[1 - c->cur], float(n.tile_threshold), inflow_on,
                         1, 2, 1, c->d_nactive};
-    hip_check(m1::f32::launch_tile_mask(ma, c->stream), "tile mask kernel");
+    {
+        Timed t(c->prof, c->stream, K_TILE_MASK);
+        hip_check(m1::f32::launch_tile_mask(ma, c->stream), "tile mask kernel");
+    }
     m1::TileZeroArgs za{dims(c), c->d_compute[c->cur], c->d_compute[1 - c->cur], c->U32,
                         c->V1_32, c->V2_32};
-    hip_check(m1::f32::launch_tile_zero(za, c->stream), "tile zero kernel");
+    {
+        Timed t(c->prof, c->stream, K_TILE_ZERO);
+        hip_check(m1::f32::launch_tile_zero(za, c->stream), "tile zero kernel");
+    }
     hip_check(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost,
                              c->stream),
               "tile count read-back");
@@ -939,7 +657,7 @@ int m1_set_problem(m1_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double dx, d
         }
         std::vector<MaterialData> mats;
         for (int m = 0; m < n_materials; ++m)
-            mats.push_back(read_material(materials[m], m));
+            mats.push_back(m1::read_material(materials[m], m));
 
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         ctx->free_problem();
@@ -958,7 +676,7 @@ int m1_set_problem(m1_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double dx, d
             double pk = 0.0;
             for (size_t i = 0; i < size_t(nx) * ny; ++i)
                 pk = std::max(pk, inflow_spatial[i]);
-            ctx->alpha = march_alpha(ctx, {pk});
+            ctx->alpha = m1::march_alpha(pk, beam_params(ctx));
         }
         const size_t m2 = size_t(nx) * ny;
         dev_alloc(&ctx->U, 4 * size_t(N), "hipMalloc state");
@@ -1164,7 +882,7 @@ int m1_spatial_operator(m1_ctx* ctx, const m1_numerics* num, double energy,
             a.out = ctx->work32;
             a.bnd = ctx->bnd32;
             {
-                Timed t(ctx, K_RHS);
+                Timed t(ctx->prof, ctx->stream, K_RHS);
                 hip_check(m1::f32::launch_stage(m1::STAGE_RHS, false, a, ctx->stream),
                           "spatial operator kernel (f32)");
             }
@@ -1180,7 +898,7 @@ int m1_spatial_operator(m1_ctx* ctx, const m1_numerics* num, double energy,
             a.U = ctx->U;
             a.out = ctx->work;
             a.bnd = ctx->bnd;
-            Timed t(ctx, K_RHS);
+            Timed t(ctx->prof, ctx->stream, K_RHS);
             hip_check(stage(ctx, n, m1::STAGE_RHS, false, a), "spatial operator kernel");
         }
         hip_check(hipMemcpyAsync(rhs, ctx->work, 4 * size_t(ctx->N) * sizeof(double),
@@ -1236,7 +954,7 @@ int m1_set_profiling(m1_ctx* ctx, int32_t enable)
 {
     return guarded(ctx, [&] {
         sync(ctx);
-        ctx->prof = enable != 0;
+        ctx->prof.on = enable != 0;
     });
 }
 
@@ -1244,7 +962,7 @@ int32_t m1_kernel_count(void) { return K_COUNT; }
 
 const char* m1_kernel_name(int32_t kernel)
 {
-    return kernel >= 0 && kernel < K_COUNT ? kKernelNames[kernel] : "";
+    return m1::kernel_name(kernel);
 }
 
 int m1_kernel_stats(m1_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms)
@@ -1254,9 +972,9 @@ int m1_kernel_stats(m1_ctx* ctx, int32_t kernel, int64_t* launches, double* tota
             throw ValueError{"kernel index out of range"};
         sync(ctx);
         if (launches)
-            *launches = ctx->launches[kernel];
+            *launches = ctx->prof.launches[kernel];
         if (total_ms)
-            *total_ms = ctx->total_ms[kernel];
+            *total_ms = ctx->prof.total_ms[kernel];
     });
 }
 
@@ -1276,10 +994,7 @@ int m1_reset_stats(m1_ctx* ctx)
         sync(ctx);
         ctx->tiles_evaluated = 0;
         ctx->tiles_full = 0;
-        for (int k = 0; k < K_COUNT; ++k) {
-            ctx->launches[k] = 0;
-            ctx->total_ms[k] = 0.0;
-        }
+        ctx->prof.reset();
     });
 }
 
