@@ -1,25 +1,28 @@
 """Benchmark of the MI355X M1 energy march (one JSON line on stdout).
 
-A step is one beamlet dose solve: a full energy march of the workload below,
-from E0 = E_beam + 5 sigma_E down to energy_min, with the problem (material
-ids, tables, inflow profile) resident in HBM before the timed region.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch256|single]
 
-Workload (the 150 MeV single-beamlet configuration of the metric): water,
-0.5 mm cells, 120 x 120 x 360 grid (6 x 6 x 18 cm), Gaussian beam sigma 4 mm,
-sigma_E 1 MeV, Wentzel-VI scattering + nuclear removal, IF-SSPRK3, WENO-Z,
-exact M1 speeds, cfl 0.9, max energy step 0.25 MeV, energy_min 1 MeV.
+Workload "batch256" (default): 256 pencil beamlets marched as one batch
+through a 200 x 200 x 300 water phantom at 1 mm holding 12 seeded ellipsoids
+(semi-axes 5-30 mm) of cortical bone (1.85 g/cm^3) or lung (0.26 g/cm^3);
+beamlet energies U(70, 230) MeV, sigma U(2, 6) mm, entry points U(-30, 30) mm,
+each beamlet on its own 64 x 64 x depth-to-range box; Wentzel-VI scattering +
+nuclear removal, IF-SSPRK3, WENO-Z, exact M1 speeds, cfl 0.9, max energy
+step 0.25 MeV, energy_min 1 MeV, FP32-bulk arithmetic with tile skipping.
+A step is one batch (256 beamlet dose solves); value is beamlets/s.  Outside
+the timed region a sample of the beamlets is marched by the plain FP64
+full-domain march (the whole phantom; bitwise the CPU oracle): its time per
+beamlet gives the FP64 beamlets/s and the speed-up, and its doses the
+max-dose error and gamma(1 %/1 mm) pass rate of the batch.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32_bulk|fp64|fp64_fast]
+Workload "single": one 150 MeV beamlet in water, 0.5 mm cells,
+120 x 120 x 360 grid, sigma 4 mm, same physics and numerics, FP32-bulk
+(--precision selects fp64 / fp64_fast); a step is one beamlet solve.
 
-The measured march is FP32-bulk by default (float reconstruction, flux and
-stage algebra; double closure, projection and dose accumulation); one plain
-FP64 solve of the same beamlet, outside the timed region, states its
-max-dose error and gamma(1%/1 mm) pass rate and the speed-up over FP64.
-
-Reported: beamlet dose solves/s (value), cell-step updates/s, the HBM
-roofline of the dominant stage kernel (algorithmic bytes per launch over its
-HIP-event duration on the march stream) and the CPU oracle timed on a bounded
-sample of the same workload on this host.
+Reported with either: cell-step updates/s, the HBM roofline of the dominant
+stage kernel (algorithmic bytes per launch over its HIP-event duration on the
+march stream) and the CPU oracle timed on a bounded sample of the same
+workload on this host.
 """
 
 from __future__ import annotations
@@ -108,25 +111,246 @@ def cpu_baseline(cfg, n_steps_full, sample_steps):
     }
 
 
+def roofline_of(stats, stage_bytes, cells_per_launch):
+    """HBM roofline of the stage kernel with the largest total time."""
+    best = None
+    for name, per_cell in stage_bytes.items():
+        launches, total_ms = stats.get(name, (0, 0.0))
+        if launches == 0:
+            continue
+        avg_s = total_ms / launches / 1000.0
+        achieved = per_cell * cells_per_launch(launches) / avg_s / 1e9
+        if best is None or total_ms > best[1]:
+            best = (name, total_ms, achieved, avg_s, per_cell, cells_per_launch(launches))
+    if best is None:
+        return None
+    name, total_ms, achieved, avg_s, per_cell, cells = best
+    return {
+        "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+        "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
+        "algorithmic_bytes_per_cell": per_cell, "cells_per_launch": round(cells),
+    }
+
+
+def batch_config(tile_threshold=1e-10):
+    from proton_solver import (
+        Beam, DeviceOptions, EnergyScheme, Geometry, Numerics, Physics, Precision, Scattering,
+        SolverConfig, SpatialScheme, WaveSpeeds,
+    )
+
+    # geometry and beam are placeholders: the batch takes the phantom and beamlets
+    return SolverConfig(
+        geometry=Geometry(size=(0.2, 0.2, 0.3), resolution=(200, 200, 300)),
+        beam=Beam(energy=150.0),
+        physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
+        numerics=Numerics(spatial_scheme=SpatialScheme.WENO_Z,
+                          energy_scheme=EnergyScheme.IF_SSPRK3, wave_speeds=WaveSpeeds.EXACT,
+                          max_energy_step=0.25, cfl=0.9, energy_min=1.0),
+        device=DeviceOptions(precision=Precision.FP32_BULK, tile_threshold=tile_threshold),
+    )
+
+
+def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_beamlet):
+    """Oracle (C/OpenMP FP64) on the first steps of one beamlet's box march."""
+    from oracle.oracle import OracleProblem
+    from proton_solver.batch import box_problem
+
+    grid, mm, inflow, beam = box_problem(phantom, box, beamlet)
+    cfg1 = type(cfg)(geometry=cfg.geometry, beam=beam, physics=cfg.physics,
+                     numerics=cfg.numerics, device=cfg.device)
+    op = OracleProblem(grid, mm, inflow, cfg1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    op.march(max_steps=1)
+    t0 = time.perf_counter()
+    _, _, E, _ = op.march(max_steps=sample_steps)
+    dt = time.perf_counter() - t0
+    steps = len(E) - 1
+    n = int(np.prod(grid.shape))
+    rate = n * steps / dt
+    return {
+        "value": rate / cell_steps_per_beamlet,
+        "unit": "beamlet dose solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"first {steps} energy steps of one {beamlet.energy:.1f} MeV beamlet on its "
+                   f"{grid.shape[0]}x{grid.shape[1]}x{grid.shape[2]} box ({dt:.1f} s, "
+                   f"{rate:.3g} cell-steps/s), extrapolated with the batch's mean box "
+                   f"cell-steps per beamlet"),
+        "cell_steps_per_s": rate,
+    }
+
+
+def run_batch(args):
+    from proton_solver import _native, metrics
+    from proton_solver.batch import (
+        batch_workload, beamlet_boxes, full_problem, prepare_batch,
+    )
+
+    cfg = batch_config(args.tile_threshold)
+    phantom, beamlets = batch_workload(n_beamlets=args.beamlets)
+    t0 = time.perf_counter()
+    boxes = beamlet_boxes(phantom, beamlets, cfg.numerics.energy_min,
+                          cfg.numerics.energy_window_sigma)
+    handle, boxes = prepare_batch(phantom, beamlets, cfg, boxes)
+    setup_s = time.perf_counter() - t0
+    num = _native.numerics_struct(cfg)
+    for _ in range(args.warmup):
+        handle.march(num, want_dose=False)
+    handle.set_profiling(True)
+    handle.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dose, steps = handle.march(num, want_dose=False)
+    elapsed = time.perf_counter() - t0
+    stats = handle.kernel_stats()
+    ev, full = handle.tile_stats()
+    host_ms = handle.host_ms()
+    handle.set_profiling(False)
+    K = max(args.steps, 1)
+    B = len(beamlets)
+    box_cells = np.array([b[2] * b[3] * b[4] for b in boxes], dtype=np.float64)
+    tile_cap = np.array([-(-b[2] // 16) * -(-b[3] // 8) * -(-b[4] // 32) * 4096 for b in boxes],
+                        dtype=np.float64)
+    fill = box_cells.sum() / tile_cap.sum()
+    cell_steps = float((box_cells * steps).sum()) * K
+    n_phantom = int(np.prod(phantom.shape))
+    eval_frac = ev / full if full else 1.0
+
+    def cells_per_launch(launches):
+        return ev * 4096.0 * fill / launches
+
+    roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch)
+
+    # Plain FP64 full-domain march of a sample of the beamlets (outside the timed region).
+    fp64 = None
+    if args.fp64_sample > 0:
+        order = np.argsort([b.energy for b in beamlets])
+        picks = [int(order[int(round(q * (B - 1)))])
+                 for q in np.linspace(0.0, 0.5, args.fp64_sample)]
+        ctx = _native.Context(0)
+        num64 = _native.numerics_struct(cfg, _native.FP64)
+        rows = []
+        for b in picks:
+            grid, mm, inflow, beam = full_problem(phantom, beamlets[b])
+            ctx.set_problem(grid, mm, inflow, beam)
+            t1 = time.perf_counter()
+            E64, _ = ctx.march(num64)
+            sec = time.perf_counter() - t1
+            ref = ctx.get_dose()
+            x0, y0, nx, ny, nz = boxes[b]
+            got = np.zeros(phantom.shape)
+            got[x0:x0 + nx, y0:y0 + ny, :nz] = handle.beamlet_dose(b)
+            # gamma on the box plus a 4-cell margin (the FP64 dose beyond it is < 1e-3 of max)
+            sl = (slice(max(x0 - 4, 0), x0 + nx + 4), slice(max(y0 - 4, 0), y0 + ny + 4),
+                  slice(0, min(nz + 4, phantom.shape[2])))
+            outside = ref.copy()
+            outside[sl] = 0.0
+            rows.append({
+                "beamlet": b, "energy_MeV": round(beamlets[b].energy, 3),
+                "fp64_seconds": round(sec, 3), "fp64_energy_steps": len(E64) - 1,
+                "batch_energy_steps": int(steps[b]),
+                "max_dose_rel_err": metrics.maximum_dose_relative_difference(got, ref),
+                "max_abs_diff_over_max": metrics.max_abs_difference_over_max(got, ref),
+                "gamma_1pct_1mm_pass_percent": metrics.gamma_pass_rate(
+                    got[sl], ref[sl], phantom.spacing, 1.0, 1.0, 10.0),
+                "fp64_dose_outside_gamma_region_over_max": float(outside.max() / ref.max()),
+            })
+            print(json.dumps({"fp64_sample": rows[-1]}), file=sys.stderr, flush=True)
+        mean_s = float(np.mean([r["fp64_seconds"] for r in rows]))
+        fp64 = {
+            "reference": "plain FP64 march of the beamlet on the whole 200x200x300 phantom "
+                         "(bitwise the CPU oracle)",
+            "samples": rows,
+            "fp64_beamlets_per_s": 1.0 / mean_s,
+            "note": "beamlets at the 0th and 50th energy percentiles of the batch; the FP64 "
+                    "time of the mean beamlet is at least the sample mean (steps grow with "
+                    "energy), so the speed-up is a lower bound",
+            "worst_max_dose_rel_err": max(abs(r["max_dose_rel_err"]) for r in rows),
+            "min_gamma_pass_percent": min(r["gamma_1pct_1mm_pass_percent"] for r in rows),
+        }
+
+    value = B * K / elapsed
+    cpu = None
+    if not args.no_cpu_baseline:
+        mid = int(np.argsort([b.energy for b in beamlets])[B // 2])
+        cpu = batch_cpu_baseline(phantom, beamlets[mid], boxes[mid], cfg, args.cpu_sample_steps,
+                                 float((box_cells * steps).sum()) / B)
+    line = {
+        "metric": "beamlet dose solves/s at <=0.5% max-dose err vs plain FP64; cell-step updates/s",
+        "value": value,
+        "unit": "beamlet dose solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk / f64 closure+projection+dose",
+        "data": "synthetic seeded phantom and beamlets (no external data)",
+        "config": {
+            "workload": "256 pencil beamlets in one batch, 200x200x300 @1 mm water phantom with "
+                        "12 bone/lung ellipsoids, 64x64xdepth boxes, Wentzel+nuclear, IF-SSPRK3, "
+                        "FP32 bulk with tile skipping",
+            "n_beamlets": B,
+            "box_cells_total": int(box_cells.sum()),
+            "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
+                                         "max": int(steps.max())},
+            "tile_threshold": args.tile_threshold,
+        },
+        "cell_steps_per_s": cell_steps / elapsed,
+        "cell_steps_per_s_evaluated": cell_steps * eval_frac / elapsed,
+        "full_domain_equivalent_cell_steps_per_s": float(n_phantom * steps.sum()) * K / elapsed,
+        "tiles_evaluated_fraction": eval_frac,
+        "host_step_setup_ms_per_batch": host_ms / K,
+        "setup_s": setup_s,
+        "kernel_ms_per_step": {k: round(v[1] / K, 3) for k, v in stats.items() if v[0]},
+        "fp64_full_domain": fp64,
+        "speedup_vs_fp64_full_domain": (value / fp64["fp64_beamlets_per_s"]) if fp64 else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="batch256", choices=["batch256", "single"])
     ap.add_argument("--precision", default="fp32_bulk", choices=["fp32_bulk", "fp64", "fp64_fast"])
     ap.add_argument("--no-fp64-reference", action="store_true",
-                    help="skip the FP64 solve used to state the error of the measured march")
-    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+                    help="skip the FP64 solves used to state the error of the measured march")
+    ap.add_argument("--cpu-sample-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--beamlets", type=int, default=256)
+    ap.add_argument("--fp64-sample", type=int, default=2)
+    ap.add_argument("--tile-threshold", type=float, default=1e-10)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one MI355X: the march of one beamlet runs on one GPU and "
-              "this build has no multi-GPU path (beamlet sharding is described in DESIGN.md).",
-              file=sys.stderr)
+        print("bench.py measures one MI355X: this build has no multi-GPU path (beamlet "
+              "sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch256":
+        args.steps = 1 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        args.cpu_sample_steps = 20 if args.cpu_sample_steps is None else args.cpu_sample_steps
+        if args.no_fp64_reference:
+            args.fp64_sample = 0
+        run_batch(args)
+        return
+    args.steps = 3 if args.steps is None else args.steps
+    args.warmup = 1 if args.warmup is None else args.warmup
+    args.cpu_sample_steps = 4 if args.cpu_sample_steps is None else args.cpu_sample_steps
+    run_single(args)
+
+
+def run_single(args):
     from proton_solver import Precision
     from proton_solver import _native
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define M1_ABI_VERSION 2
+#define M1_ABI_VERSION 3
 
 enum {
     M1_OK = 0,
@@ -184,6 +184,64 @@ int m1_reset_stats(m1_ctx *ctx);
 /* Tiles evaluated by the stage kernels since the last m1_reset_stats, and the
  * number a full-domain march would have evaluated. */
 int m1_tile_stats(m1_ctx *ctx, int64_t *evaluated, int64_t *full);
+
+/* ------------------------------------------------------------------
+ * Batched beamlets.  No reference counterpart: the reference solves one
+ * beam per energy_march call (solver.py:102-114).  A batch holds a
+ * phantom (material ids on a regular grid) and many pencil beamlets that
+ * enter at the phantom's z-min face; beamlet b is marched on its own box
+ * of the phantom, cells [x0, x0+nx) x [y0, y0+ny) x [0, nz), with the
+ * method of energy_march applied to that box as its domain (its own
+ * energy grid from the CFL condition over the box, zero-gradient
+ * boundaries on the box faces, inflow on its z-min face).  All boxes are
+ * resident in device memory and each stage launch advances the evaluated
+ * tiles of every beamlet still marching.  The batch runs the FP32-bulk
+ * kernels (precision must be M1_FP32_BULK) with the IF-SSPRK3 scheme, or
+ * SSPRK3 without scattering and nuclear removal.
+ * ------------------------------------------------------------------ */
+typedef struct m1_batch m1_batch;
+
+typedef struct {
+    int32_t x0, y0;        /* box origin in phantom cells */
+    int32_t nx, ny, nz;    /* box extent in cells; the box starts at z = 0 */
+    int32_t reserved;      /* 0 */
+    m1_beam beam;
+} m1_beamlet;
+
+int m1_batch_create(m1_batch **out, int device);
+void m1_batch_destroy(m1_batch *batch);
+const char *m1_batch_last_error(const m1_batch *batch);
+
+/* Phantom (nx, ny, nz) with cell sizes (dx, dy, dz) and ids in
+ * [0, n_materials); beamlets[b] with its transverse inflow profile at
+ * inflow_spatial + sum_{c<b} nx_c*ny_c, shape (nx_b, ny_b) (the
+ * BeamInflow.spatial of the beam on the box grid, boundary.py:68-80). */
+int m1_batch_set_problem(m1_batch *batch, int32_t nx, int32_t ny, int32_t nz,
+                         double dx, double dy, double dz, const int32_t *ids,
+                         int32_t n_materials, const m1_material *materials,
+                         int32_t n_beamlets, const m1_beamlet *beamlets,
+                         const double *inflow_spatial);
+
+/* March every beamlet from E_b + energy_window_sigma * sigma_E down to
+ * energy_min.  dose (phantom shape, Gy; may be NULL) receives the sum of
+ * the beamlet doses; n_steps (n_beamlets; may be NULL) the number of energy
+ * steps of each beamlet. */
+int m1_batch_march(m1_batch *batch, const m1_numerics *num, double *dose,
+                   int32_t *n_steps);
+
+/* Dose of beamlet b from the last march on its box, (nx_b, ny_b, nz_b), Gy. */
+int m1_batch_beamlet_dose(m1_batch *batch, int32_t beamlet, double *dose);
+
+/* Kernel timing and tile counts, as m1_set_profiling / m1_kernel_stats /
+ * m1_tile_stats / m1_reset_stats. */
+int m1_batch_set_profiling(m1_batch *batch, int32_t enable);
+int m1_batch_kernel_stats(m1_batch *batch, int32_t kernel, int64_t *launches,
+                          double *total_ms);
+int m1_batch_tile_stats(m1_batch *batch, int64_t *evaluated, int64_t *full);
+int m1_batch_reset_stats(m1_batch *batch);
+/* Host time of the per-step physics scalars since the last reset, ms. */
+int m1_batch_host_ms(m1_batch *batch, double *setup_ms);
+
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,8 @@ def _units():
          ["-DM1_EXACT=0", "-ffp-contract=fast"]),
         (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=fast"]),
         (CSRC / "m1_march.hip", BUILD / "m1_march.o", ["-ffp-contract=off"]),
+        (CSRC / "m1_batch_kernels.hip", BUILD / "m1_batch_kernels.o", ["-ffp-contract=fast"]),
+        (CSRC / "m1_batch_march.hip", BUILD / "m1_batch_march.o", ["-ffp-contract=off"]),
     ]
 
 

@@ -1,0 +1,344 @@
+// Batched-beamlet kernels for gfx950: one launch advances the evaluated tiles
+// of every beamlet box of the batch.
+//
+// The per-tile arithmetic is the FP32-bulk stage body of m1_f32_common.h, so
+// a beamlet marched in the batch follows the same arithmetic as the same box
+// marched alone.  What is batch-specific is the work distribution: a stage
+// launch runs one 256-thread workgroup per entry of a compacted, ascending
+// list of (box, tile) pairs, and workgroups are mapped so that each of the 8
+// XCDs (one L2 each) takes one contiguous slice of the list, keeping the
+// halo re-reads of neighbouring tiles in the same L2.
+#include "m1_f32_common.h"
+
+namespace m1 {
+namespace batch {
+namespace {
+
+using namespace f32::detail;
+using f32::TileView;
+
+constexpr int XCDS = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// List slot of workgroup i: XCD x = i % 8 receives the slots
+// [x q + min(x, r), ...) with n = 8 q + r, in order.
+__device__ __forceinline__ int xcd_slot(int i, int n)
+{
+    const int q = n / XCDS, r = n % XCDS;
+    const int x = i % XCDS, j = i / XCDS;
+    return x * q + min(x, r) + j;
+}
+
+struct TileCoord {
+    int b, tx, ty, tz;
+};
+
+__device__ __forceinline__ TileCoord tile_coord(const BoxDesc& d, int b, int t)
+{
+    const int local = t - d.tile_off;
+    return {b, local / (d.ntz * d.nty), (local / d.ntz) % d.nty, local % d.ntz};
+}
+
+template <int KIND, bool IFAC>
+__global__ void __launch_bounds__(NT) stage_kernel(BatchStageArgs a)
+{
+    const int t = a.list[xcd_slot(int(blockIdx.x), a.n_list)];
+    const int b = a.tile_box[t];
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    const long off4 = 4 * d.cell_off;
+    const long m2 = long(d.nx) * d.ny;
+    TileView v{{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz},
+               a.src + off4,
+               a.U + off4,
+               a.V1 ? a.V1 + off4 : nullptr,
+               a.out ? a.out + off4 : nullptr,
+               a.dose_hi ? a.dose_hi + d.cell_off : nullptr,
+               a.dose_lo ? a.dose_lo + d.cell_off : nullptr,
+               nullptr,
+               a.ids + d.cell_off,
+               a.bnd + 12 * d.col_off + 4 * m2 * a.bnd_stage,
+               a.coef + b,
+               a.tile_max ? a.tile_max + t : nullptr,
+               a.wave_mode,
+               tc.tx,
+               tc.ty,
+               tc.tz};
+    stage_tile<KIND, IFAC>(v);
+}
+
+// CFL maximum per box over all its cells.  Tiles outside the live set hold
+// the exact zero state: their cells take the vacuum value of their material
+// (and of the inflow on z-min cells) without reading the state.
+__global__ void __launch_bounds__(NT) cfl_kernel(BatchCflArgs a)
+{
+    __shared__ double red[NT / 64];
+    const int t = blockIdx.x;
+    const int b = a.tile_box[t];
+    if (!a.step[b].active)
+        return;
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    const StepCoef& c = a.coef[b];
+    const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
+    const long N = long(d.nx) * d.ny * d.nz;
+    const double eps = c.eig_eps;
+    const double pin_scale = c.ed[0] * c.inv_alpha, rf = c.rf;
+    const uint8_t* ids = a.ids + d.cell_off;
+    const double* spatial = a.spatial + d.col_off;
+    const int tid = threadIdx.x;
+    const int x0 = tc.tx * TX, y0 = tc.ty * TY, z0 = tc.tz * TZ;
+    const int x1 = min(x0 + TX, d.nx);
+    double mx = 0.0;
+    bool bad = false;
+    if (!a.live || a.live[t]) {
+        const float* U = a.U + 4 * d.cell_off;
+        const int z = z0 + tid % TZ, y = y0 + tid / TZ;
+        if (z < d.nz && y < d.ny) {
+            for (int x = x0; x < x1; ++x) {
+                const long i = (long(x) * d.ny + y) * d.nz + z;
+                const bool zmin = z == 0;
+                const double pin = zmin ? spatial[i / d.nz] * pin_scale : 0.0;
+                const double v = cfl_cell(U, i, N, c.S0[ids[i]], g, a.wave_mode, eps, zmin, pin, rf);
+                if (!(v == v))
+                    bad = true;
+                else
+                    mx = fmax(mx, v);
+            }
+        }
+    } else {
+        const unsigned mats = a.tile_mats[t];
+        if (tid < MAXMAT && (mats >> tid & 1u))
+            mx = cfl_vacuum(c.S0[tid], g, a.wave_mode, eps, false, 0.0, rf);
+        if (tc.tz == 0 && tid < TX * TY) {
+            const int x = x0 + tid / TY, y = y0 + tid % TY;
+            if (x < d.nx && y < d.ny) {
+                const long col = long(x) * d.ny + y;
+                const double pin = spatial[col] * pin_scale;
+                mx = fmax(mx, cfl_vacuum(c.S0[ids[col * d.nz]], g, a.wave_mode, eps, true, pin,
+                                         rf));
+            }
+        }
+    }
+    block_max_to(mx, bad, red, a.out + b);
+}
+
+__global__ void boundary_kernel(BatchBoundaryArgs a)
+{
+    const int b = blockIdx.y;
+    if (!a.step[b].active)
+        return;
+    const BoxDesc d = a.box[b];
+    const long m2 = long(d.nx) * d.ny;
+    const long col = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= m2)
+        return;
+    boundary_column(a.coef[b], a.ids[d.cell_off + col * d.nz], a.spatial[d.col_off + col], col, m2,
+                    a.integrating_factor != 0, a.bnd + 12 * d.col_off);
+}
+
+// Next-step tile set of every active box: the single-box rule (a tile is
+// evaluated when a tile within the dilation radius holds phi above the
+// threshold, or it is a z = 0 tile while the inflow is on) per box.
+__global__ void mask_kernel(BatchMaskArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_tiles)
+        return;
+    const int b = a.tile_box[t];
+    const BoxStep st = a.step[b];
+    if (!st.active) {
+        a.next[t] = 0;
+        return;
+    }
+    if (a.all_on) {
+        a.next[t] = 1;
+        return;
+    }
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    bool on = st.inflow_on && tc.tz == 0;
+    for (int dx = -a.rx; dx <= a.rx && !on; ++dx) {
+        const int xx = tc.tx + dx;
+        if (xx < 0 || xx >= d.ntx)
+            continue;
+        for (int dy = -a.ry; dy <= a.ry && !on; ++dy) {
+            const int yy = tc.ty + dy;
+            if (yy < 0 || yy >= d.nty)
+                continue;
+            for (int dz = -a.rz; dz <= a.rz && !on; ++dz) {
+                const int zz = tc.tz + dz;
+                if (zz < 0 || zz >= d.ntz)
+                    continue;
+                const int u = d.tile_off + (xx * d.nty + yy) * d.ntz + zz;
+                on = a.computed[u] && a.tile_max[u] > a.threshold;
+            }
+        }
+    }
+    a.next[t] = on ? 1 : 0;
+}
+
+// One workgroup: ascending list of the set flags (each thread scans one
+// contiguous chunk; chunk counts are prefix-summed in LDS).
+constexpr int COMPACT_THREADS = 1024;
+
+__global__ void __launch_bounds__(COMPACT_THREADS)
+    compact_kernel(const uint8_t* flags, int n, int* list, int* count)
+{
+    __shared__ int sums[COMPACT_THREADS];
+    const int tid = threadIdx.x;
+    const int per = (n + COMPACT_THREADS - 1) / COMPACT_THREADS;
+    const int lo = min(tid * per, n), hi = min(lo + per, n);
+    int c = 0;
+    for (int i = lo; i < hi; ++i)
+        c += flags[i] != 0;
+    sums[tid] = c;
+    __syncthreads();
+    for (int off = 1; off < COMPACT_THREADS; off <<= 1) {
+        const int v = tid >= off ? sums[tid - off] : 0;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    int pos = sums[tid] - c;
+    for (int i = lo; i < hi; ++i)
+        if (flags[i])
+            list[pos++] = i;
+    if (tid == COMPACT_THREADS - 1)
+        *count = sums[tid];
+}
+
+// Tiles that leave the evaluated set become exact vacuum (U = V1 = V2 = 0).
+__global__ void __launch_bounds__(NT) zero_kernel(BatchZeroArgs a)
+{
+    const int t = blockIdx.x;
+    if (!(a.computed[t] && !a.next[t]))
+        return;
+    const int b = a.tile_box[t];
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    const long N = long(d.nx) * d.ny * d.nz;
+    const int z = tc.tz * TZ + threadIdx.x % TZ;
+    const int y = tc.ty * TY + threadIdx.x / TZ;
+    if (z >= d.nz || y >= d.ny)
+        return;
+    const long off4 = 4 * d.cell_off;
+    for (int x = tc.tx * TX; x < min(tc.tx * TX + TX, d.nx); ++x) {
+        const long i = (long(x) * d.ny + y) * d.nz + z;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            a.U[off4 + i + c * N] = 0.f;
+            a.V1[off4 + i + c * N] = 0.f;
+            a.V2[off4 + i + c * N] = 0.f;
+        }
+    }
+}
+
+__global__ void scatter_dose_kernel(const float* hi, const float* lo, int nx, int ny, int nz,
+                                    int x0, int y0, int py, int pz, double* phantom)
+{
+    const long n = long(nx) * ny * nz;
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int z = int(i % nz);
+    const int y = int((i / nz) % ny);
+    const int x = int(i / (long(nz) * ny));
+    phantom[(long(x0 + x) * py + (y0 + y)) * pz + z] += double(hi[i]) + double(lo[i]);
+}
+
+__global__ void box_dose_kernel(const float* hi, const float* lo, long n, double* out)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = double(hi[i]) + double(lo[i]);
+}
+
+}  // namespace
+
+hipError_t launch_stage(int kind, bool ifac, const BatchStageArgs& a, hipStream_t s)
+{
+    if (a.n_list <= 0)
+        return hipSuccess;
+    const dim3 grid(unsigned(a.n_list)), block(NT);
+#define M1_LAUNCH(K, I) hipLaunchKernelGGL((stage_kernel<K, I>), grid, block, 0, s, a)
+    switch (kind) {
+    case STAGE0:
+        M1_LAUNCH(STAGE0, false);
+        break;
+    case STAGE1:
+        if (ifac)
+            M1_LAUNCH(STAGE1, true);
+        else
+            M1_LAUNCH(STAGE1, false);
+        break;
+    case STAGE2:
+        if (ifac)
+            M1_LAUNCH(STAGE2, true);
+        else
+            M1_LAUNCH(STAGE2, false);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef M1_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl(const BatchCflArgs& a, hipStream_t s)
+{
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(cfl_kernel, dim3(unsigned(a.n_tiles)), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary(const BatchBoundaryArgs& a, int max_columns, hipStream_t s)
+{
+    if (a.n_boxes <= 0 || max_columns <= 0)
+        return hipSuccess;
+    const dim3 grid(unsigned((max_columns + 255) / 256), unsigned(a.n_boxes));
+    hipLaunchKernelGGL(boundary_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask(const BatchMaskArgs& a, hipStream_t s)
+{
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(mask_kernel, dim3(unsigned((a.n_tiles + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const uint8_t* flags, int n, int* list, int* count, hipStream_t s)
+{
+    hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(COMPACT_THREADS), 0, s, flags, n, list,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(const BatchZeroArgs& a, hipStream_t s)
+{
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(a.n_tiles)), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_dose(const float* hi, const float* lo, int nx, int ny, int nz, int x0,
+                               int y0, int py, int pz, double* phantom, hipStream_t s)
+{
+    const long n = long(nx) * ny * nz;
+    hipLaunchKernelGGL(scatter_dose_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, hi,
+                       lo, nx, ny, nz, x0, y0, py, pz, phantom);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_dose(const float* hi, const float* lo, long n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(box_dose_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, hi, lo,
+                       n, out);
+    return hipGetLastError();
+}
+
+}  // namespace batch
+}  // namespace m1

@@ -1,0 +1,651 @@
+// Host side of the batched-beamlet march (m1_batch_* in include/m1march.h).
+//
+// Each beamlet is the energy march of energy_march (numerics/energy.py:
+// 1497-1644) on its own box of the phantom: its own energy E_b and CFL step,
+// its own inflow, its own tile set.  One iteration of the driver advances
+// every beamlet that is still marching by one of its energy steps:
+//   1. one CFL launch over the tiles of all active boxes (per-box maxima,
+//      read back together with the size of this step's tile list);
+//   2. per-beamlet step scalars on the host (the single-box code, per box);
+//   3. one boundary launch and three stage launches over the compacted
+//      (box, tile) list;
+//   4. per-box tile sets of the next step (mask, compaction, zeroing).
+// The host synchronises once per iteration (step 1's read-back).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "m1_host.h"
+
+using namespace m1;
+
+namespace {
+std::string g_batch_create_error;
+}  // namespace
+
+struct m1_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    bool have_problem = false;
+    int px = 0, py = 0, pz = 0;
+    double dx = 0, dy = 0, dz = 0;
+    std::vector<MaterialData> mats;
+    std::vector<m1_beamlet> bl;
+    std::vector<BoxDesc> boxes;
+    std::vector<double> alpha;
+    long total_cells = 0, total_cols = 0, max_box_cells = 0;
+    int total_tiles = 0, max_cols = 0;
+
+    BoxDesc* d_box = nullptr;
+    BoxStep* d_step = nullptr;
+    int* d_tile_box = nullptr;
+    uint8_t* d_tile_mats = nullptr;
+    float *U = nullptr, *V1 = nullptr, *V2 = nullptr, *dose_hi = nullptr, *dose_lo = nullptr,
+          *bnd = nullptr, *d_tile_max = nullptr;
+    uint8_t* ids = nullptr;
+    double* spatial = nullptr;
+    uint8_t* d_flags[2] = {nullptr, nullptr};
+    int* d_list = nullptr;
+    int* d_count = nullptr;
+    StepCoef* d_coef = nullptr;
+    StepCoef* d_cfl = nullptr;
+    StepCoef32* d_coef32 = nullptr;
+    unsigned long long* d_red = nullptr;
+    double* d_phantom = nullptr;
+    double* d_box_dose = nullptr;
+
+    StepCoef* h_coef = nullptr;
+    StepCoef* h_cfl = nullptr;
+    StepCoef32* h_coef32 = nullptr;
+    BoxStep* h_step = nullptr;
+    unsigned long long* h_red = nullptr;
+    int* h_count = nullptr;
+
+    Profiler prof;
+    double host_ms = 0.0;
+    bool marched = false;
+
+    void free_problem();
+    ~m1_batch();
+};
+
+void m1_batch::free_problem()
+{
+    void* dev[] = {d_box,  d_step,   d_tile_box, d_tile_mats, U,         V1,       V2,
+                   dose_hi, dose_lo, bnd,        d_tile_max,  ids,       spatial,  d_flags[0],
+                   d_flags[1], d_list, d_count,  d_coef,      d_cfl,     d_coef32, d_red,
+                   d_phantom, d_box_dose};
+    for (void* p : dev)
+        if (p)
+            (void)hipFree(p);
+    void* host[] = {h_coef, h_cfl, h_coef32, h_step, h_red, h_count};
+    for (void* p : host)
+        if (p)
+            (void)hipHostFree(p);
+    d_box = nullptr;
+    d_step = nullptr;
+    d_tile_box = nullptr;
+    d_tile_mats = nullptr;
+    U = V1 = V2 = dose_hi = dose_lo = bnd = d_tile_max = nullptr;
+    ids = nullptr;
+    spatial = nullptr;
+    d_flags[0] = d_flags[1] = nullptr;
+    d_list = d_count = nullptr;
+    d_coef = d_cfl = nullptr;
+    d_coef32 = nullptr;
+    d_red = nullptr;
+    d_phantom = d_box_dose = nullptr;
+    h_coef = h_cfl = nullptr;
+    h_coef32 = nullptr;
+    h_step = nullptr;
+    h_red = nullptr;
+    h_count = nullptr;
+    have_problem = false;
+    marched = false;
+}
+
+m1_batch::~m1_batch()
+{
+    (void)hipSetDevice(device);
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    free_problem();
+    prof.release();
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+template <class F>
+int guarded(m1_batch* c, F&& f)
+{
+    if (!c)
+        return M1_ERR_STATE;
+    return run_guarded(c->device, c->err, f);
+}
+
+void sync(m1_batch* c)
+{
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (c->prof.on)
+        c->prof.collect();
+}
+
+BeamParams beam_params(const m1_beamlet& b)
+{
+    return {b.beam.energy, b.beam.energy_spread, b.beam.angular_spread};
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+        .count();
+}
+
+// z = 0 tiles of a box stay evaluated while the inflow spectrum, anywhere in
+// the next two maximal steps, exceeds the threshold relative to its peak.
+int inflow_on(const m1_beamlet& b, const Num& n, double energy_next)
+{
+    const double lo = energy_next - 2.0 * n.max_step;
+    const double e = std::min(std::max(b.beam.energy, lo), energy_next);
+    const double d = (e - b.beam.energy) / b.beam.energy_spread;
+    return std::exp(-0.5 * d * d) > n.tile_threshold ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int m1_batch_create(m1_batch** out, int device)
+{
+    if (!out) {
+        g_batch_create_error = "out must not be NULL";
+        return M1_ERR_VALUE;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        g_batch_create_error = "no HIP device is available";
+        return M1_ERR_HIP;
+    }
+    if (device < 0 || device >= count) {
+        g_batch_create_error = "device ordinal out of range";
+        return M1_ERR_VALUE;
+    }
+    m1_batch* c = new m1_batch;
+    c->device = device;
+    try {
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    } catch (const HipError& e) {
+        g_batch_create_error = e.msg;
+        delete c;
+        return M1_ERR_HIP;
+    }
+    *out = c;
+    return M1_OK;
+}
+
+void m1_batch_destroy(m1_batch* batch) { delete batch; }
+
+const char* m1_batch_last_error(const m1_batch* batch)
+{
+    return batch ? batch->err.c_str() : g_batch_create_error.c_str();
+}
+
+int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double dx, double dy,
+                         double dz, const int32_t* ids, int32_t n_materials,
+                         const m1_material* materials, int32_t n_beamlets,
+                         const m1_beamlet* beamlets, const double* inflow_spatial)
+{
+    return guarded(c, [&] {
+        if (nx <= 0 || ny <= 0 || nz <= 0)
+            throw ValueError{"Geometry resolution must contain positive integers."};
+        if (!(dx > 0 && dy > 0 && dz > 0))
+            throw ValueError{"cell sizes must be positive"};
+        if (n_materials < 1 || n_materials > MAXMAT)
+            throw ValueError{"between 1 and 8 materials are supported"};
+        if (n_beamlets < 1)
+            throw ValueError{"a batch needs at least one beamlet"};
+        if (!ids || !materials || !beamlets || !inflow_spatial)
+            throw ValueError{"ids, materials, beamlets and inflow must not be NULL"};
+        const long P = long(nx) * ny * nz;
+        for (long i = 0; i < P; ++i)
+            if (ids[i] < 0 || ids[i] >= n_materials)
+                throw ValueError{"material id out of range"};
+        std::vector<MaterialData> mats;
+        for (int m = 0; m < n_materials; ++m)
+            mats.push_back(read_material(materials[m], m));
+
+        std::vector<BoxDesc> boxes(static_cast<size_t>(n_beamlets));
+        std::vector<double> alpha(static_cast<size_t>(n_beamlets));
+        long cells = 0, cols = 0, max_cells = 0;
+        long tiles = 0;
+        int max_cols = 0;
+        for (int b = 0; b < n_beamlets; ++b) {
+            const m1_beamlet& q = beamlets[b];
+            if (q.nx <= 0 || q.ny <= 0 || q.nz <= 0)
+                throw ValueError{"beamlet " + std::to_string(b) + ": empty box"};
+            if (q.nz < 4)
+                throw ValueError{"WENO-Z inflow extrapolation requires at least four z cells."};
+            if (q.x0 < 0 || q.y0 < 0 || q.x0 + q.nx > nx || q.y0 + q.ny > ny || q.nz > nz)
+                throw ValueError{"beamlet " + std::to_string(b) + ": box outside the phantom"};
+            if (q.beam.energy_spread == 0.0)
+                throw ValueError{"Zero energy spread is not supported by the Gaussian inflow."};
+            if (!std::isfinite(q.beam.energy) || !std::isfinite(q.beam.energy_spread) ||
+                !std::isfinite(q.beam.angular_spread))
+                throw ValueError{"Beam parameters must be finite."};
+            BoxDesc& d = boxes[size_t(b)];
+            d.nx = q.nx;
+            d.ny = q.ny;
+            d.nz = q.nz;
+            d.ntx = (q.nx + F32_TX - 1) / F32_TX;
+            d.nty = (q.ny + F32_TY - 1) / F32_TY;
+            d.ntz = (q.nz + F32_TZ - 1) / F32_TZ;
+            d.tile_off = int(tiles);
+            d.pad = 0;
+            d.cell_off = cells;
+            d.col_off = cols;
+            const long n = long(q.nx) * q.ny * q.nz;
+            const long m2 = long(q.nx) * q.ny;
+            double pk = 0.0;
+            for (long i = 0; i < m2; ++i) {
+                const double v = inflow_spatial[cols + i];
+                if (!(std::isfinite(v) && v >= 0.0))
+                    throw ValueError{"inflow profile values must be finite and non-negative"};
+                pk = std::max(pk, v);
+            }
+            alpha[size_t(b)] = march_alpha(pk, beam_params(q));
+            cells += n;
+            cols += m2;
+            tiles += long(d.ntx) * d.nty * d.ntz;
+            max_cells = std::max(max_cells, n);
+            max_cols = std::max(max_cols, int(m2));
+            if (tiles > (1L << 30))
+                throw ValueError{"batch too large"};
+        }
+
+        // Per-box material ids and per-tile material sets.
+        std::vector<uint8_t> box_ids(static_cast<size_t>(cells));
+        std::vector<int> tile_box(static_cast<size_t>(tiles));
+        std::vector<uint8_t> tile_mats(static_cast<size_t>(tiles), 0);
+        for (int b = 0; b < n_beamlets; ++b) {
+            const m1_beamlet& q = beamlets[b];
+            const BoxDesc& d = boxes[size_t(b)];
+            for (int x = 0; x < q.nx; ++x)
+                for (int y = 0; y < q.ny; ++y) {
+                    const int32_t* src = ids + (long(q.x0 + x) * ny + (q.y0 + y)) * nz;
+                    uint8_t* dst = box_ids.data() + d.cell_off + (long(x) * q.ny + y) * q.nz;
+                    for (int z = 0; z < q.nz; ++z) {
+                        dst[z] = uint8_t(src[z]);
+                        const int t = d.tile_off +
+                                      ((x / F32_TX) * d.nty + y / F32_TY) * d.ntz + z / F32_TZ;
+                        tile_mats[size_t(t)] |= uint8_t(1u << src[z]);
+                    }
+                }
+            for (int t = 0; t < d.ntx * d.nty * d.ntz; ++t)
+                tile_box[size_t(d.tile_off + t)] = b;
+        }
+
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->free_problem();
+        c->px = nx;
+        c->py = ny;
+        c->pz = nz;
+        c->dx = dx;
+        c->dy = dy;
+        c->dz = dz;
+        c->mats = std::move(mats);
+        c->bl.assign(beamlets, beamlets + n_beamlets);
+        c->boxes = boxes;
+        c->alpha = alpha;
+        c->total_cells = cells;
+        c->total_cols = cols;
+        c->total_tiles = int(tiles);
+        c->max_cols = max_cols;
+        c->max_box_cells = max_cells;
+        const size_t B = size_t(n_beamlets), T = size_t(tiles), N = size_t(cells);
+        dev_alloc(&c->d_box, B, "hipMalloc boxes");
+        dev_alloc(&c->d_step, B, "hipMalloc box flags");
+        dev_alloc(&c->d_tile_box, T, "hipMalloc tile boxes");
+        dev_alloc(&c->d_tile_mats, T, "hipMalloc tile materials");
+        dev_alloc(&c->U, 4 * N, "hipMalloc batch state");
+        dev_alloc(&c->V1, 4 * N, "hipMalloc batch stage 1");
+        dev_alloc(&c->V2, 4 * N, "hipMalloc batch stage 2");
+        dev_alloc(&c->dose_hi, N, "hipMalloc batch dose");
+        dev_alloc(&c->dose_lo, N, "hipMalloc batch dose");
+        dev_alloc(&c->bnd, 12 * size_t(cols), "hipMalloc batch boundary");
+        dev_alloc(&c->d_tile_max, T, "hipMalloc tile max");
+        dev_alloc(&c->ids, N, "hipMalloc batch ids");
+        dev_alloc(&c->spatial, size_t(cols), "hipMalloc batch inflow");
+        dev_alloc(&c->d_flags[0], T, "hipMalloc tile set");
+        dev_alloc(&c->d_flags[1], T, "hipMalloc tile set");
+        dev_alloc(&c->d_list, T, "hipMalloc tile list");
+        dev_alloc(&c->d_count, 1, "hipMalloc tile count");
+        dev_alloc(&c->d_coef, B, "hipMalloc batch coefficients");
+        dev_alloc(&c->d_cfl, B, "hipMalloc batch CFL coefficients");
+        dev_alloc(&c->d_coef32, B, "hipMalloc batch coefficients (f32)");
+        dev_alloc(&c->d_red, B, "hipMalloc batch CFL maxima");
+        dev_alloc(&c->d_box_dose, size_t(max_cells), "hipMalloc beamlet dose");
+        host_alloc(&c->h_coef, B, "hipHostMalloc");
+        host_alloc(&c->h_cfl, B, "hipHostMalloc");
+        host_alloc(&c->h_coef32, B, "hipHostMalloc");
+        host_alloc(&c->h_step, B, "hipHostMalloc");
+        host_alloc(&c->h_red, B, "hipHostMalloc");
+        host_alloc(&c->h_count, 1, "hipHostMalloc");
+        hip_check(hipMemcpy(c->d_box, boxes.data(), B * sizeof(BoxDesc), hipMemcpyHostToDevice),
+                  "box upload");
+        hip_check(hipMemcpy(c->d_tile_box, tile_box.data(), T * sizeof(int),
+                            hipMemcpyHostToDevice),
+                  "tile upload");
+        hip_check(hipMemcpy(c->d_tile_mats, tile_mats.data(), T, hipMemcpyHostToDevice),
+                  "tile upload");
+        hip_check(hipMemcpy(c->ids, box_ids.data(), N, hipMemcpyHostToDevice), "ids upload");
+        hip_check(hipMemcpy(c->spatial, inflow_spatial, size_t(cols) * sizeof(double),
+                            hipMemcpyHostToDevice),
+                  "inflow upload");
+        c->have_problem = true;
+    });
+}
+
+int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n_steps)
+{
+    return guarded(c, [&] {
+        if (!c->have_problem)
+            throw ValueError{"no problem bound: call m1_batch_set_problem first"};
+        const Num n = read_numerics(num);
+        if (n.precision != M1_FP32_BULK)
+            throw NotImplemented{"the batched march runs the FP32-bulk kernels (M1_FP32_BULK)"};
+        bool ifac;
+        if (n.scheme == M1_IF_SSPRK3)
+            ifac = has_sources(n);
+        else if (n.scheme == M1_SSPRK3 && !has_sources(n))
+            ifac = false;
+        else
+            throw NotImplemented{"the batched march supports IF-SSPRK3, and SSPRK3 without "
+                                 "scattering or nuclear removal"};
+        const int B = int(c->bl.size());
+        const size_t N = size_t(c->total_cells);
+        const size_t T = size_t(c->total_tiles);
+        hipStream_t s = c->stream;
+
+        const size_t nb = static_cast<size_t>(B);
+        std::vector<double> E(nb), dE(nb, 0.0);
+        std::vector<int32_t> steps(nb, 0);
+        std::vector<int> tiles_of(nb);
+        int n_active = 0;
+        for (int b = 0; b < B; ++b) {
+            E[size_t(b)] = c->bl[size_t(b)].beam.energy +
+                           n.window * c->bl[size_t(b)].beam.energy_spread;
+            const BoxDesc& d = c->boxes[size_t(b)];
+            tiles_of[size_t(b)] = d.ntx * d.nty * d.ntz;
+            const bool on = E[size_t(b)] > n.emin;
+            c->h_step[b] = {on ? 1 : 0, 1};
+            n_active += on ? 1 : 0;
+        }
+        for (float* p : {c->U, c->V1, c->V2})
+            hip_check(hipMemsetAsync(p, 0, 4 * N * sizeof(float), s), "state reset");
+        hip_check(hipMemsetAsync(c->dose_hi, 0, N * sizeof(float), s), "dose reset");
+        hip_check(hipMemsetAsync(c->dose_lo, 0, N * sizeof(float), s), "dose reset");
+        // The first step evaluates every tile.
+        int cur = 0;
+        hip_check(hipMemsetAsync(c->d_flags[cur], 1, T, s), "tile set");
+        hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
+                                 hipMemcpyHostToDevice, s),
+                  "box flag upload");
+        {
+            // Boxes with nothing to march leave the list at once.
+            BatchMaskArgs ma{c->d_box, c->d_step, c->d_tile_box, int(T), c->d_tile_max,
+                             c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1};
+            hip_check(batch::launch_mask(ma, s), "tile mask kernel");
+            cur = 1 - cur;
+            hip_check(batch::launch_compact(c->d_flags[cur], int(T), c->d_list, c->d_count, s),
+                      "tile compaction kernel");
+        }
+        const size_t nm = c->mats.size();
+        const bool skipping = n.tile_threshold > 0.0;
+
+        while (n_active > 0) {
+            // 1. CFL maxima of every active box.
+            auto t0 = std::chrono::steady_clock::now();
+            for (int b = 0; b < B; ++b)
+                if (c->h_step[b].active) {
+                    fill_cfl(c->mats, beam_params(c->bl[size_t(b)]), E[size_t(b)], c->h_cfl[b]);
+                    set_norm(c->alpha[size_t(b)], c->h_cfl[b]);
+                }
+            c->host_ms += ms_since(t0);
+            hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, size_t(B) * sizeof(StepCoef),
+                                     hipMemcpyHostToDevice, s),
+                      "CFL coefficient upload");
+            hip_check(hipMemsetAsync(c->d_red, 0, size_t(B) * sizeof(unsigned long long), s),
+                      "CFL reset");
+            {
+                BatchCflArgs a{c->d_box,  c->d_step,        c->d_tile_box, int(T), c->dx,
+                               c->dy,     c->dz,            c->U,          c->ids, c->spatial,
+                               c->d_flags[cur], c->d_tile_mats, c->d_cfl,  c->d_red, n.wave};
+                Timed t(c->prof, s, K_CFL);
+                hip_check(batch::launch_cfl(a, s), "CFL kernel (batch)");
+            }
+            hip_check(hipMemcpyAsync(c->h_red, c->d_red, size_t(B) * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, s),
+                      "CFL read-back");
+            hip_check(hipMemcpyAsync(c->h_count, c->d_count, sizeof(int), hipMemcpyDeviceToHost,
+                                     s),
+                      "tile count read-back");
+            sync(c);
+            const int n_list = *c->h_count;
+            c->prof.tiles_evaluated += n_list;
+
+            // 2. Step scalars of every active beamlet.
+            t0 = std::chrono::steady_clock::now();
+            for (int b = 0; b < B; ++b) {
+                if (!c->h_step[b].active)
+                    continue;
+                double mx;
+                std::memcpy(&mx, &c->h_red[b], sizeof mx);
+                double d = cfl_energy_step(n, mx);
+                if (E[size_t(b)] - n.emin < d)
+                    d = E[size_t(b)] - n.emin;
+                dE[size_t(b)] = d;
+                fill_step(c->mats, beam_params(c->bl[size_t(b)]), n, E[size_t(b)], d, ifac,
+                          false, c->h_coef[b]);
+                set_norm(c->alpha[size_t(b)], c->h_coef[b]);
+                fill_step32(nm, c->alpha[size_t(b)], c->h_coef[b], c->h_coef32[b]);
+                c->prof.tiles_full += tiles_of[size_t(b)];
+            }
+            c->host_ms += ms_since(t0);
+            hip_check(hipMemcpyAsync(c->d_coef, c->h_coef, size_t(B) * sizeof(StepCoef),
+                                     hipMemcpyHostToDevice, s),
+                      "coefficient upload");
+            hip_check(hipMemcpyAsync(c->d_coef32, c->h_coef32, size_t(B) * sizeof(StepCoef32),
+                                     hipMemcpyHostToDevice, s),
+                      "coefficient upload (f32)");
+
+            // 3. Boundary traces and the three stages over the tile list.
+            {
+                BatchBoundaryArgs ba{c->d_box, c->d_step, B,       c->spatial,
+                                     c->ids,   c->d_coef, c->bnd, ifac ? 1 : 0};
+                Timed t(c->prof, s, K_BOUNDARY);
+                hip_check(batch::launch_boundary(ba, c->max_cols, s), "boundary kernel (batch)");
+            }
+            BatchStageArgs a{};
+            a.box = c->d_box;
+            a.tile_box = c->d_tile_box;
+            a.list = c->d_list;
+            a.n_list = n_list;
+            a.dx = c->dx;
+            a.dy = c->dy;
+            a.dz = c->dz;
+            a.U = c->U;
+            a.V1 = c->V1;
+            a.ids = c->ids;
+            a.bnd = c->bnd;
+            a.coef = c->d_coef32;
+            a.tile_max = skipping ? c->d_tile_max : nullptr;
+            a.wave_mode = n.wave;
+            {
+                a.src = c->U;
+                a.out = c->V1;
+                a.bnd_stage = 0;
+                Timed t(c->prof, s, K_STAGE0);
+                hip_check(batch::launch_stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
+            }
+            {
+                a.src = c->V1;
+                a.out = c->V2;
+                a.bnd_stage = 1;
+                Timed t(c->prof, s, K_STAGE1);
+                hip_check(batch::launch_stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
+            }
+            {
+                a.src = c->V2;
+                a.out = nullptr;
+                a.dose_hi = c->dose_hi;
+                a.dose_lo = c->dose_lo;
+                a.bnd_stage = 2;
+                Timed t(c->prof, s, K_STAGE2);
+                hip_check(batch::launch_stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
+            }
+
+            // 4. Energies and the next step's tile sets.
+            n_active = 0;
+            for (int b = 0; b < B; ++b) {
+                if (!c->h_step[b].active)
+                    continue;
+                E[size_t(b)] -= dE[size_t(b)];
+                steps[size_t(b)] += 1;
+                const bool on = E[size_t(b)] > n.emin;
+                c->h_step[b].active = on ? 1 : 0;
+                c->h_step[b].inflow_on = on ? inflow_on(c->bl[size_t(b)], n, E[size_t(b)]) : 0;
+                n_active += on ? 1 : 0;
+            }
+            if (n_active == 0)
+                break;
+            hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
+                                     hipMemcpyHostToDevice, s),
+                      "box flag upload");
+            {
+                BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
+                                 int(T),        c->d_tile_max,   c->d_flags[cur],
+                                 c->d_flags[1 - cur], float(n.tile_threshold), 1, 2, 1,
+                                 skipping ? 0 : 1};
+                Timed t(c->prof, s, K_TILE_MASK);
+                hip_check(batch::launch_mask(ma, s), "tile mask kernel (batch)");
+                hip_check(batch::launch_compact(c->d_flags[1 - cur], int(T), c->d_list,
+                                                c->d_count, s),
+                          "tile compaction kernel");
+            }
+            {
+                BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags[cur],
+                                 c->d_flags[1 - cur], c->U, c->V1, c->V2};
+                Timed t(c->prof, s, K_TILE_ZERO);
+                hip_check(batch::launch_zero(za, s), "tile zero kernel (batch)");
+            }
+            cur = 1 - cur;
+        }
+
+        if (dose) {
+            const size_t P = size_t(c->px) * c->py * c->pz;
+            if (!c->d_phantom)
+                dev_alloc(&c->d_phantom, P, "hipMalloc phantom dose");
+            hip_check(hipMemsetAsync(c->d_phantom, 0, P * sizeof(double), s), "dose reset");
+            {
+                Timed t(c->prof, s, K_DOSE);
+                for (int b = 0; b < B; ++b) {
+                    const m1_beamlet& q = c->bl[size_t(b)];
+                    const BoxDesc& d = c->boxes[size_t(b)];
+                    hip_check(batch::launch_scatter_dose(c->dose_hi + d.cell_off,
+                                                         c->dose_lo + d.cell_off, q.nx, q.ny, q.nz,
+                                                         q.x0, q.y0, c->py, c->pz, c->d_phantom, s),
+                              "dose scatter kernel");
+                }
+            }
+            hip_check(hipMemcpyAsync(dose, c->d_phantom, P * sizeof(double),
+                                     hipMemcpyDeviceToHost, s),
+                      "dose download");
+        }
+        sync(c);
+        if (n_steps)
+            std::memcpy(n_steps, steps.data(), size_t(B) * sizeof(int32_t));
+        c->marched = true;
+    });
+}
+
+int m1_batch_beamlet_dose(m1_batch* c, int32_t beamlet, double* dose)
+{
+    return guarded(c, [&] {
+        if (!c->have_problem)
+            throw ValueError{"no problem bound: call m1_batch_set_problem first"};
+        if (beamlet < 0 || beamlet >= int(c->bl.size()))
+            throw ValueError{"beamlet index out of range"};
+        if (!dose)
+            throw ValueError{"dose must not be NULL"};
+        if (!c->marched)
+            throw ValueError{"no march has run on this batch"};
+        const BoxDesc& d = c->boxes[size_t(beamlet)];
+        const long n = long(d.nx) * d.ny * d.nz;
+        hip_check(batch::launch_box_dose(c->dose_hi + d.cell_off, c->dose_lo + d.cell_off, n,
+                                         c->d_box_dose, c->stream),
+                  "beamlet dose kernel");
+        hip_check(hipMemcpyAsync(dose, c->d_box_dose, size_t(n) * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream),
+                  "dose download");
+        sync(c);
+    });
+}
+
+int m1_batch_set_profiling(m1_batch* c, int32_t enable)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->prof.on = enable != 0;
+    });
+}
+
+int m1_batch_kernel_stats(m1_batch* c, int32_t kernel, int64_t* launches, double* total_ms)
+{
+    return guarded(c, [&] {
+        if (kernel < 0 || kernel >= K_COUNT)
+            throw ValueError{"kernel index out of range"};
+        sync(c);
+        if (launches)
+            *launches = c->prof.launches[kernel];
+        if (total_ms)
+            *total_ms = c->prof.total_ms[kernel];
+    });
+}
+
+int m1_batch_tile_stats(m1_batch* c, int64_t* evaluated, int64_t* full)
+{
+    return guarded(c, [&] {
+        if (evaluated)
+            *evaluated = c->prof.tiles_evaluated;
+        if (full)
+            *full = c->prof.tiles_full;
+    });
+}
+
+int m1_batch_reset_stats(m1_batch* c)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->prof.reset();
+        c->host_ms = 0.0;
+    });
+}
+
+int m1_batch_host_ms(m1_batch* c, double* setup_ms)
+{
+    return guarded(c, [&] {
+        if (setup_ms)
+            *setup_ms = c->host_ms;
+    });
+}
+
+}  // extern "C"

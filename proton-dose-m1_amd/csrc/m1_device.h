@@ -175,6 +175,114 @@ struct BoundaryArgs32 {
     int integrating_factor;
 };
 
+// ------------------------------------------------ batched beamlets (FP32 bulk)
+// Every beamlet marches in its own box of the phantom.  The boxes' arrays are
+// concatenated: box b owns cells [cell_off, cell_off + n_b) of every per-cell
+// array (the state holds 4 component planes of n_b cells from 4 * cell_off),
+// columns [col_off, col_off + nx_b ny_b) of the inflow profile (boundary
+// traces: 12 floats per column from 12 * col_off) and the workgroup tiles
+// [tile_off, tile_off + ntx ntz nty) of the global tile index space.
+struct BoxDesc {
+    int nx, ny, nz;
+    int ntx, nty, ntz;
+    int tile_off;
+    int pad;
+    long cell_off;
+    long col_off;
+};
+
+// Per box and step, written by the host.
+struct BoxStep {
+    int active;     // the beamlet is still marching
+    int inflow_on;  // z = 0 tiles stay evaluated: the inflow spectrum is on
+};
+
+struct BatchStageArgs {
+    const BoxDesc* box;
+    const int* tile_box;     // global tile -> box
+    const int* list;         // tiles to evaluate (global tile index), ascending
+    int n_list;
+    double dx, dy, dz;
+    const float* src;
+    float* U;
+    const float* V1;
+    float* out;
+    float* dose_hi;
+    float* dose_lo;
+    const uint8_t* ids;
+    const float* bnd;
+    int bnd_stage;           // boundary traces of stage 0, 1 or 2
+    const StepCoef32* coef;  // per box
+    float* tile_max;         // per global tile (nullable)
+    int wave_mode;
+};
+
+struct BatchCflArgs {
+    const BoxDesc* box;
+    const BoxStep* step;
+    const int* tile_box;
+    int n_tiles;
+    double dx, dy, dz;
+    const float* U;
+    const uint8_t* ids;
+    const double* spatial;
+    const uint8_t* live;       // per tile: state may be non-zero (else exact vacuum)
+    const uint8_t* tile_mats;  // per tile: bit m set when material m occurs in it
+    const StepCoef* coef;      // per box: S0, ed[0], rf, inv_alpha, eig_eps
+    unsigned long long* out;   // per box, ordered bits of the maximum
+    int wave_mode;
+};
+
+struct BatchBoundaryArgs {
+    const BoxDesc* box;
+    const BoxStep* step;
+    int n_boxes;
+    const double* spatial;
+    const uint8_t* ids;
+    const StepCoef* coef;      // per box
+    float* bnd;
+    int integrating_factor;
+};
+
+struct BatchMaskArgs {
+    const BoxDesc* box;
+    const BoxStep* step;
+    const int* tile_box;
+    int n_tiles;
+    const float* tile_max;
+    const uint8_t* computed;
+    uint8_t* next;
+    float threshold;
+    int rx, ry, rz;
+    int all_on;               // no skipping: every tile of an active box
+};
+
+struct BatchZeroArgs {
+    const BoxDesc* box;
+    const int* tile_box;
+    int n_tiles;
+    const uint8_t* computed;
+    const uint8_t* next;
+    float* U;
+    float* V1;
+    float* V2;
+};
+
+namespace batch {
+hipError_t launch_stage(int kind, bool integrating_factor, const BatchStageArgs& a,
+                        hipStream_t s);
+hipError_t launch_cfl(const BatchCflArgs& a, hipStream_t s);
+hipError_t launch_boundary(const BatchBoundaryArgs& a, int max_columns, hipStream_t s);
+hipError_t launch_mask(const BatchMaskArgs& a, hipStream_t s);
+// list = ascending indices i < n with flags[i] != 0; *count = their number
+hipError_t launch_compact(const uint8_t* flags, int n, int* list, int* count, hipStream_t s);
+hipError_t launch_zero(const BatchZeroArgs& a, hipStream_t s);
+// phantom[x0 + x, y0 + y, z] += hi + lo over the box (px, py, pz phantom shape)
+hipError_t launch_scatter_dose(const float* hi, const float* lo, int nx, int ny, int nz, int x0,
+                               int y0, int py, int pz, double* phantom, hipStream_t s);
+hipError_t launch_box_dose(const float* hi, const float* lo, long n, double* out, hipStream_t s);
+}  // namespace batch
+
 namespace f32 {
 hipError_t launch_stage(int kind, bool integrating_factor, const StageArgs32& a, hipStream_t s);
 hipError_t launch_cfl(const CflArgs32& a, hipStream_t s);

@@ -235,6 +235,131 @@ __device__ __forceinline__ float zmin_ghost(int k, double uin, double u0, double
     return float(g);
 }
 
+__device__ __forceinline__ double amax(double lo, double hi) { return fmax(fabs(lo), fabs(hi)); }
+
+// Inverse CFL step of one cell of the scaled float state (energy.py:539-660):
+// sum over axes of max|lambda| / (S d_axis).  A z-min cell also counts the
+// inflow state (p_in = scaled inflow phi, J = (0, 0, rf p_in)) on the z axis.
+__device__ __forceinline__ double cfl_cell(const float* U, long i, long N, double s,
+                                           const GridDims& g, int wave_mode, double eps,
+                                           bool zmin, double p_in, double rf)
+{
+    if (wave_mode == 1)
+        return 1.0 / (s * g.dx) + 1.0 / (s * g.dy) + 1.0 / (s * g.dz);
+    const double phi = U[i];
+    const double J[3] = {U[i + N], U[i + 2 * N], U[i + 3 * N]};
+    const double jn = sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+    Closure c = closure(phi, J, jn);
+    double lo, hi, ax, ay, az;
+    eigen<0>(phi, c, eps, lo, hi);
+    ax = amax(lo, hi);
+    eigen<1>(phi, c, eps, lo, hi);
+    ay = amax(lo, hi);
+    eigen<2>(phi, c, eps, lo, hi);
+    az = amax(lo, hi);
+    if (zmin) {
+        const double Jin[3] = {0.0, 0.0, rf * p_in};
+        const Closure ci = closure(p_in, Jin, fabs(Jin[2]));
+        eigen<2>(p_in, ci, eps, lo, hi);
+        az = fmax(az, amax(lo, hi));
+    }
+    return ax / (s * g.dx) + ay / (s * g.dy) + az / (s * g.dz);
+}
+
+// Inverse CFL step of a cell holding the exact zero state: the isotropic
+// speeds 1/sqrt(3) on every axis (wave_speeds.py:37-50), the same value
+// cfl_cell returns for U = 0.
+__device__ __forceinline__ double cfl_vacuum(double s, const GridDims& g, int wave_mode, double eps,
+                                             bool zmin, double p_in, double rf)
+{
+    const double iso = 0.57735026918962576;
+    if (wave_mode == 1)
+        return 1.0 / (s * g.dx) + 1.0 / (s * g.dy) + 1.0 / (s * g.dz);
+    double az = iso;
+    if (zmin) {
+        double lo, hi;
+        const double Jin[3] = {0.0, 0.0, rf * p_in};
+        const Closure ci = closure(p_in, Jin, fabs(Jin[2]));
+        eigen<2>(p_in, ci, eps, lo, hi);
+        az = fmax(az, amax(lo, hi));
+    }
+    return iso / (s * g.dx) + iso / (s * g.dy) + az / (s * g.dz);
+}
+
+// Workgroup maximum of mx (NaN if any thread saw one) folded into *out as
+// ordered bits (non-negative doubles order like their bit patterns; NaN
+// sorts above every finite value).  Needs blockDim.x / 64 slots of `red`.
+__device__ __forceinline__ void block_max_to(double mx, bool bad, double* red,
+                                             unsigned long long* out)
+{
+    if (bad)
+        mx = __longlong_as_double(0x7ff8000000000000LL);
+    unsigned long long bits = __double_as_longlong(mx);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = __longlong_as_double(bits);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = 0;
+        for (int w = 0; w < int(blockDim.x / 64); ++w) {
+            const unsigned long long o = __double_as_longlong(red[w]);
+            b = o > b ? o : b;
+        }
+        atomicMax(out, b);
+    }
+}
+
+// Stage boundary traces of one z-min column b (energy.py:107-230, 348-455) in
+// double, stored scaled by 1/alpha in float: bnd holds 3 x (4, m2) values for
+// stages 0, 1, 2.  m is the material of the column's z = 0 cell and sp its
+// inflow profile value.
+__device__ __forceinline__ void boundary_column(const StepCoef& c, int m, double sp, long b,
+                                                long m2, bool integrating_factor, float* bnd)
+{
+    const double ia = c.inv_alpha;
+    float* o0 = bnd;
+    float* o1 = bnd + 4 * m2;
+    float* o2 = bnd + 8 * m2;
+    const double p0 = sp * c.ed[0];
+    o0[b] = float(p0 * ia);
+    o0[m2 + b] = 0.f;
+    o0[2 * m2 + b] = 0.f;
+    o0[3 * m2 + b] = float(c.rf * p0 * ia);
+    const double Sk[4] = {c.S0[m], c.S13[m], c.S23[m], c.S1[m]};
+    double B[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double p = sp * c.ed[k];
+        B[k][0] = Sk[k] * p;
+        B[k][1] = 0.0;
+        B[k][2] = 0.0;
+        B[k][3] = Sk[k] * (c.rf * p);
+    }
+    if (integrating_factor) {
+        const double ip[3] = {c.i13p[m], c.i23p[m], c.i1p[m]};
+        const double ij[3] = {c.i13j[m], c.i23j[m], c.i1j[m]};
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            B[k][0] *= ip[k - 1];
+            B[k][3] *= ij[k - 1];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double v1 = 0.5 * (-9.0 * B[0][q] + 18.0 * B[1][q] - 9.0 * B[2][q] + 2.0 * B[3][q]);
+        double v2 = 0.25 * (11.0 * B[0][q] - 27.0 * B[1][q] + 27.0 * B[2][q] - 7.0 * B[3][q]);
+        if (integrating_factor) {
+            v1 *= (q == 0 ? c.f1p[m] : c.f1j[m]);
+            v2 *= (q == 0 ? c.f2p[m] : c.f2j[m]);
+        }
+        o1[q * m2 + b] = float(v1 / c.S1[m] * ia);
+        o2[q * m2 + b] = float(v2 / c.S2[m] * ia);
+    }
+}
+
 // Compensated accumulation hi + lo += v (Knuth TwoSum).
 __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
 {

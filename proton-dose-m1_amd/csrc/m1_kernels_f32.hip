@@ -40,8 +40,6 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
 // ---------------------------------------------------------------- CFL
 constexpr int CFL_THREADS = 256;
 
-__device__ __forceinline__ double amax(double lo, double hi) { return fmax(fabs(lo), fabs(hi)); }
-
 __global__ void __launch_bounds__(CFL_THREADS) cfl_kernel(CflArgs32 a)
 {
     __shared__ double S0[MAXMAT];
@@ -53,113 +51,31 @@ __global__ void __launch_bounds__(CFL_THREADS) cfl_kernel(CflArgs32 a)
     const long N = long(nx) * ny * nz;
     const double eps = a.coef->eig_eps;
     const double pin_scale = a.coef->ed[0] * a.coef->inv_alpha, rf = a.coef->rf;
-    const double iso = 0.57735026918962576;
     double mx = 0.0;
     bool bad = false;
     for (long i = long(blockIdx.x) * CFL_THREADS + threadIdx.x; i < N;
          i += long(gridDim.x) * CFL_THREADS) {
-        const double s = S0[a.ids[i]];
-        double v;
-        if (a.wave_mode == 1) {
-            v = 1.0 / (s * a.g.dx) + 1.0 / (s * a.g.dy) + 1.0 / (s * a.g.dz);
-        } else {
-            const double phi = a.U[i];
-            const double J[3] = {a.U[i + N], a.U[i + 2 * N], a.U[i + 3 * N]};
-            const double jn = sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
-            Closure c = closure(phi, J, jn);
-            double lo, hi, ax, ay, az;
-            eigen<0>(phi, c, eps, lo, hi);
-            ax = amax(lo, hi);
-            eigen<1>(phi, c, eps, lo, hi);
-            ay = amax(lo, hi);
-            eigen<2>(phi, c, eps, lo, hi);
-            az = amax(lo, hi);
-            if (i % nz == 0) {
-                const double p = a.spatial[i / nz] * pin_scale;
-                const double Jin[3] = {0.0, 0.0, rf * p};
-                const Closure ci = closure(p, Jin, fabs(Jin[2]));
-                eigen<2>(p, ci, eps, lo, hi);
-                az = fmax(az, amax(lo, hi));
-            }
-            (void)iso;
-            v = ax / (s * a.g.dx) + ay / (s * a.g.dy) + az / (s * a.g.dz);
-        }
+        const bool zmin = i % nz == 0;
+        const double pin = zmin ? a.spatial[i / nz] * pin_scale : 0.0;
+        const double v = cfl_cell(a.U, i, N, S0[a.ids[i]], a.g, a.wave_mode, eps, zmin, pin, rf);
         if (!(v == v))
             bad = true;
         else
             mx = fmax(mx, v);
     }
-    if (bad)
-        mx = __longlong_as_double(0x7ff8000000000000LL);
-    unsigned long long bits = __double_as_longlong(mx);
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(bits, off, 64);
-        bits = o > bits ? o : bits;
-    }
-    if ((threadIdx.x & 63) == 0)
-        red[threadIdx.x >> 6] = __longlong_as_double(bits);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long b = 0;
-        for (int w = 0; w < CFL_THREADS / 64; ++w) {
-            const unsigned long long o = __double_as_longlong(red[w]);
-            b = o > b ? o : b;
-        }
-        atomicMax(a.out, b);
-    }
+    block_max_to(mx, bad, red, a.out);
 }
 
 // Stage boundary traces (energy.py:107-230, 348-455) in double, stored
 // scaled by 1/alpha in float.
 __global__ void boundary_kernel(BoundaryArgs32 a)
 {
-    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-    const long m2 = long(nx) * ny;
+    const long m2 = long(a.g.nx) * a.g.ny;
     const long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= m2)
         return;
-    const StepCoef& c = *a.coef;
-    const int m = a.ids[b * nz];
-    const double sp = a.spatial[b];
-    const double ia = c.inv_alpha;
-    float* o0 = a.bnd;
-    float* o1 = a.bnd + 4 * m2;
-    float* o2 = a.bnd + 8 * m2;
-    const double p0 = sp * c.ed[0];
-    o0[b] = float(p0 * ia);
-    o0[m2 + b] = 0.f;
-    o0[2 * m2 + b] = 0.f;
-    o0[3 * m2 + b] = float(c.rf * p0 * ia);
-    const double Sk[4] = {c.S0[m], c.S13[m], c.S23[m], c.S1[m]};
-    double B[4][4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double p = sp * c.ed[k];
-        B[k][0] = Sk[k] * p;
-        B[k][1] = 0.0;
-        B[k][2] = 0.0;
-        B[k][3] = Sk[k] * (c.rf * p);
-    }
-    if (a.integrating_factor) {
-        const double ip[3] = {c.i13p[m], c.i23p[m], c.i1p[m]};
-        const double ij[3] = {c.i13j[m], c.i23j[m], c.i1j[m]};
-#pragma unroll
-        for (int k = 1; k < 4; ++k) {
-            B[k][0] *= ip[k - 1];
-            B[k][3] *= ij[k - 1];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        double v1 = 0.5 * (-9.0 * B[0][q] + 18.0 * B[1][q] - 9.0 * B[2][q] + 2.0 * B[3][q]);
-        double v2 = 0.25 * (11.0 * B[0][q] - 27.0 * B[1][q] + 27.0 * B[2][q] - 7.0 * B[3][q]);
-        if (a.integrating_factor) {
-            v1 *= (q == 0 ? c.f1p[m] : c.f1j[m]);
-            v2 *= (q == 0 ? c.f2p[m] : c.f2j[m]);
-        }
-        o1[q * m2 + b] = float(v1 / c.S1[m] * ia);
-        o2[q * m2 + b] = float(v2 / c.S2[m] * ia);
-    }
+    boundary_column(*a.coef, a.ids[b * a.g.nz], a.spatial[b], b, m2, a.integrating_factor != 0,
+                    a.bnd);
 }
 
 __global__ void finalize_dose_kernel(const float* hi, const float* lo, double* dose, long n)

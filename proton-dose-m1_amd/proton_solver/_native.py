@@ -16,7 +16,7 @@ import numpy as np
 from .enums import CoefficientInterpolation, EnergyScheme, Precision, Scattering, WaveSpeeds
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libm1march.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 M1_OK = 0
 M1_ERR_VALUE = -1
@@ -57,6 +57,11 @@ class m1_numerics(C.Structure):
                 ("tile_threshold", C.c_double)]
 
 
+class m1_beamlet(C.Structure):
+    _fields_ = [("x0", C.c_int32), ("y0", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("nz", C.c_int32), ("reserved", C.c_int32), ("beam", m1_beam)]
+
+
 # (name, restype, argtypes) of every exported function.
 SIGNATURES = [
     ("m1_abi_version", C.c_int, []),
@@ -83,6 +88,22 @@ SIGNATURES = [
                                   C.POINTER(C.c_double)]),
     ("m1_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_tile_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("m1_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("m1_batch_destroy", None, [C.c_void_p]),
+    ("m1_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("m1_batch_set_problem", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                       C.c_double, C.c_double, _ip, C.c_int32,
+                                       C.POINTER(m1_material), C.c_int32,
+                                       C.POINTER(m1_beamlet), _dp]),
+    ("m1_batch_march", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), _dp, _ip]),
+    ("m1_batch_beamlet_dose", C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    ("m1_batch_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_kernel_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_double)]),
+    ("m1_batch_tile_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64)]),
+    ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
+    ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
 ]
 
 _lib = None
@@ -309,5 +330,113 @@ class Context:
             n = C.c_int64()
             ms = C.c_double()
             self._check(self.lib.m1_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
+            out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
+        return out
+
+
+class Batch:
+    """One libm1march batch: a phantom and its beamlet boxes in device memory."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.m1_batch_create(C.byref(h), int(device))
+        if rc != M1_OK:
+            raise MarchError(self.lib.m1_batch_last_error(None).decode())
+        self.h = h
+        self.shape = None
+        self.boxes = ()
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.m1_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc == M1_OK:
+            return
+        msg = self.lib.m1_batch_last_error(self.h).decode()
+        if rc == M1_ERR_VALUE:
+            raise ValueError(msg)
+        if rc == M1_ERR_NOT_IMPLEMENTED:
+            raise NotImplementedError(msg)
+        raise MarchError(msg)
+
+    def set_problem(self, ids, spacing, materials, boxes, profiles, beams):
+        """ids (nx, ny, nz) int; spacing (dx, dy, dz) m; materials: one per id;
+        boxes[b] = (x0, y0, nx_b, ny_b, nz_b); profiles[b] (nx_b, ny_b) inflow
+        profile; beams[b] with energy, energy_spread, angular_spread."""
+        ids = np.ascontiguousarray(np.asarray(ids), dtype=np.int32)
+        if ids.ndim != 3:
+            raise ValueError("ids must be a 3-D array")
+        if not (len(boxes) == len(profiles) == len(beams)):
+            raise ValueError("boxes, profiles and beams must have one entry per beamlet")
+        keep = []
+        mats = (m1_material * len(materials))()
+        for i, m in enumerate(materials):
+            mats[i] = material_struct(m, keep)
+        bl = (m1_beamlet * len(boxes))()
+        flat = []
+        for i, (box, prof, beam) in enumerate(zip(boxes, profiles, beams)):
+            x0, y0, nx, ny, nz = (int(v) for v in box)
+            prof = np.asarray(prof, dtype=np.float64)
+            if prof.shape != (nx, ny):
+                raise ValueError(f"inflow profile {i} must have shape ({nx}, {ny})")
+            bl[i] = m1_beamlet(x0, y0, nx, ny, nz, 0,
+                               m1_beam(float(beam.energy), float(beam.energy_spread),
+                                       float(beam.angular_spread)))
+            flat.append(prof.ravel())
+        spatial = np.ascontiguousarray(np.concatenate(flat), dtype=np.float64)
+        dx, dy, dz = (float(v) for v in spacing)
+        self._check(self.lib.m1_batch_set_problem(
+            self.h, *(int(v) for v in ids.shape), dx, dy, dz, _ptr(ids), len(materials), mats,
+            len(boxes), bl, _ptr(spatial)))
+        self.shape = tuple(int(v) for v in ids.shape)
+        self.boxes = tuple(tuple(int(v) for v in b) for b in boxes)
+
+    def march(self, num: m1_numerics, want_dose: bool = True):
+        """Returns (phantom dose or None, energy steps per beamlet)."""
+        dose = np.empty(self.shape, dtype=np.float64) if want_dose else None
+        steps = np.zeros(len(self.boxes), dtype=np.int32)
+        self._check(self.lib.m1_batch_march(self.h, C.byref(num), _ptr(dose), _ptr(steps)))
+        return dose, steps
+
+    def beamlet_dose(self, b: int):
+        if not 0 <= int(b) < len(self.boxes):
+            raise ValueError("beamlet index out of range")
+        x0, y0, nx, ny, nz = self.boxes[b]
+        out = np.empty((nx, ny, nz), dtype=np.float64)
+        self._check(self.lib.m1_batch_beamlet_dose(self.h, int(b), _ptr(out)))
+        return out
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.m1_batch_set_profiling(self.h, int(bool(on))))
+
+    def reset_stats(self):
+        self._check(self.lib.m1_batch_reset_stats(self.h))
+
+    def tile_stats(self):
+        a = C.c_int64()
+        b = C.c_int64()
+        self._check(self.lib.m1_batch_tile_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def host_ms(self):
+        v = C.c_double()
+        self._check(self.lib.m1_batch_host_ms(self.h, C.byref(v)))
+        return v.value
+
+    def kernel_stats(self):
+        out = {}
+        for k in range(self.lib.m1_kernel_count()):
+            n = C.c_int64()
+            ms = C.c_double()
+            self._check(self.lib.m1_batch_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
             out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
         return out

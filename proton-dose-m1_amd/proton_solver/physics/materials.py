@@ -30,8 +30,42 @@ def _material_rows():
     return rows
 
 
+@dataclass(frozen=True)
+class ScaledMaterial:
+    """A repository material at `density_scale` times its tabulated density.
+
+    No reference counterpart (the reference's materials are the five tabulated
+    ones).  At fixed composition the linear stopping power and the transport
+    and removal coefficients are proportional to density, so their tables are
+    the base tables times `density_scale`, and the radiation length is divided
+    by it.  Used for lung at 0.26 g/cm^3 (the tabulated lung is 1.04).
+    """
+
+    base: Material
+    density_scale: float
+
+    def __post_init__(self):
+        if not isinstance(self.base, Material):
+            raise ValueError("base must be a Material.")
+        if not (np.isfinite(self.density_scale) and self.density_scale > 0.0):
+            raise ValueError("density_scale must be finite and positive.")
+
+    @property
+    def value(self) -> str:
+        return f"{self.base.value}*{self.density_scale:g}"
+
+
+def table_material(material):
+    """(tabulated material, factor on its linear coefficients)."""
+    if isinstance(material, ScaledMaterial):
+        return material.base, float(material.density_scale)
+    return material, 1.0
+
+
 def density(material: Material) -> float:
     """Material density in kg/m^3 (materials.py:29-38)."""
+    if isinstance(material, ScaledMaterial):
+        return density(material.base) * material.density_scale
     row = _material_rows().get(material.value)
     if row is None:
         raise ValueError(f"Unknown material: {material.value}")
@@ -40,6 +74,8 @@ def density(material: Material) -> float:
 
 def radiation_length(material: Material) -> float:
     """Radiation length in metres (materials.py:40-63)."""
+    if isinstance(material, ScaledMaterial):
+        return radiation_length(material.base) / material.density_scale
     row = _material_rows().get(material.value)
     if row is None:
         raise ValueError(f"Unknown material: {material.value}")

@@ -9,7 +9,7 @@ import numpy as np
 
 from ..enums import CoefficientInterpolation, Material
 from .coefficient_interpolation import check_in_table, interpolate_coefficient, pchip_slopes
-from .materials import MaterialMap
+from .materials import MaterialMap, ScaledMaterial
 from .stopping_power import _read_table, stopping_power
 
 DATA_DIR = Path(__file__).resolve().parents[2] / "data" / "nuclear_attenuation"
@@ -18,6 +18,9 @@ DATA_DIR = Path(__file__).resolve().parents[2] / "data" / "nuclear_attenuation"
 @lru_cache
 def load_nuclear_table(material: Material):
     """(energies [MeV], Sigma_r [1/m]) with the checks of nuclear.py:30-81."""
+    if isinstance(material, ScaledMaterial):
+        energies, values = load_nuclear_table(material.base)
+        return energies, values * material.density_scale
     path = DATA_DIR / f"{material.value}.csv"
     if not path.exists():
         raise FileNotFoundError(f"No nuclear-attenuation data found for {material.value}.")

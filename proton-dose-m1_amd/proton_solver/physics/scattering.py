@@ -7,7 +7,7 @@ import numpy as np
 
 from ..enums import CoefficientInterpolation, Material, Scattering
 from .coefficient_interpolation import check_in_table, interpolate_coefficient, pchip_slopes
-from .materials import MaterialMap, radiation_length
+from .materials import MaterialMap, ScaledMaterial, radiation_length
 from .stopping_power import _read_table, stopping_power
 
 DATA_DIR = Path(__file__).resolve().parents[2] / "data" / "wentzel_transport"
@@ -19,6 +19,9 @@ ROSSI_ES_MEV = 15.0
 @lru_cache
 def load_transport_table(material: Material):
     """(energies [MeV], Sigma_tr [1/m]) of the Wentzel-VI table."""
+    if isinstance(material, ScaledMaterial):
+        energies, values = load_transport_table(material.base)
+        return energies, values * material.density_scale
     return _read_table(
         DATA_DIR / f"{material.value}.csv",
         ("energy_mev", "first_transport_coefficient_per_m"),

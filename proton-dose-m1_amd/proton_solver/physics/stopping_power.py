@@ -7,7 +7,7 @@ import numpy as np
 
 from ..enums import Material
 from .coefficient_interpolation import check_in_table, pchip_evaluate, pchip_slopes
-from .materials import MaterialMap
+from .materials import MaterialMap, ScaledMaterial
 
 DATA_DIR = Path(__file__).resolve().parents[2] / "data" / "stopping_power"
 
@@ -24,6 +24,9 @@ def _read_table(path: Path, columns: tuple[str, ...]):
 @lru_cache
 def load_stopping_power_table(material: Material):
     """(energies [MeV], S [MeV/m]) of one material."""
+    if isinstance(material, ScaledMaterial):
+        energies, values = load_stopping_power_table(material.base)
+        return energies, values * material.density_scale
     path = DATA_DIR / f"{material.value}.csv"
     if not path.exists():
         raise FileNotFoundError(f"No stopping-power data found for {material.value}.")

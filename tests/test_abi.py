@@ -52,6 +52,8 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_native.m1_beam) == 24
     assert ctypes.sizeof(_native.m1_numerics) == 6 * 4 + 5 * 8
     assert _native.m1_numerics.max_energy_step.offset == 24
+    assert ctypes.sizeof(_native.m1_beamlet) == 6 * 4 + 24
+    assert _native.m1_beamlet.beam.offset == 24
 
 
 def test_library_kernel_names_without_gpu():
