@@ -54,8 +54,11 @@ enum { M1_INTERP_LINEAR = 0, M1_INTERP_PCHIP = 1 };
  * reference's operation order with every operation separately rounded (the
  * FP64 parity mode); M1_FP64_FAST lets the compiler fuse multiply-adds and
  * multiplies by reciprocal constants; M1_FP32_BULK is the mixed-precision
- * march. */
-enum { M1_FP64 = 0, M1_FP32_BULK = 1, M1_FP64_FAST = 2 };
+ * march (float state, reconstruction, closure, eigenvalues and fluxes, with
+ * 1 - |J|^2/phi^2 formed exactly as |J|/phi -> 1, inflow extrapolation and
+ * compensated dose in double / float pairs); M1_FP32_BULK_F64_CLOSURE is the
+ * same with face projection, closure, eigenvalues and HLL in double. */
+enum { M1_FP64 = 0, M1_FP32_BULK = 1, M1_FP64_FAST = 2, M1_FP32_BULK_F64_CLOSURE = 3 };
 /* m1_step kinds: energy.py:663 forward_euler, 725 ssprk3, 878 if_ssprk3,
  * 1194 strang_energy_step. */
 enum { M1_STEP_FORWARD_EULER = 0, M1_STEP_SSPRK3 = 1, M1_STEP_IF_SSPRK3 = 2,
@@ -98,12 +101,12 @@ typedef struct {
     int32_t scattering;                 /* M1_SCATTERING_* */
     int32_t nuclear;                    /* 0 / 1 */
     int32_t coefficient_interpolation;  /* M1_INTERP_* */
-    int32_t precision;                  /* M1_FP64 / M1_FP32_BULK / M1_FP64_FAST */
+    int32_t precision;                  /* M1_FP64 / M1_FP32_BULK / ... */
     double max_energy_step;             /* MeV */
     double cfl;
     double energy_min;                  /* MeV */
     double energy_window_sigma;
-    /* FP32_BULK only: workgroup tiles (16 x 8 x 32 cells) whose 9-cell
+    /* FP32_BULK modes only: workgroup tiles (16 x 8 x 32 cells) whose 9-cell
      * neighbourhood holds phi below tile_threshold x (peak inflow fluence)
      * are not evaluated and are kept as exact vacuum; 0 evaluates every
      * tile.  No reference counterpart (the reference's closest analogue is

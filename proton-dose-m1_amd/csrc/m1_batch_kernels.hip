@@ -38,7 +38,7 @@ __device__ __forceinline__ TileCoord tile_coord(const BoxDesc& d, int b, int t)
     return {b, local / (d.ntz * d.nty), (local / d.ntz) % d.nty, local % d.ntz};
 }
 
-template <int KIND, bool IFAC>
+template <int KIND, bool IFAC, bool D>
 __global__ void __launch_bounds__(NT) stage_kernel(BatchStageArgs a)
 {
     const int t = a.list[xcd_slot(int(blockIdx.x), a.n_list)];
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(BatchStageArgs a)
                tc.tx,
                tc.ty,
                tc.tz};
-    stage_tile<KIND, IFAC>(v);
+    stage_tile<KIND, IFAC, D>(v);
 }
 
 // CFL maximum per box over all its cells.  Tiles outside the live set hold
@@ -260,7 +260,13 @@ hipError_t launch_stage(int kind, bool ifac, const BatchStageArgs& a, hipStream_
     if (a.n_list <= 0)
         return hipSuccess;
     const dim3 grid(unsigned(a.n_list)), block(NT);
-#define M1_LAUNCH(K, I) hipLaunchKernelGGL((stage_kernel<K, I>), grid, block, 0, s, a)
+#define M1_LAUNCH(K, I)                                                                   \
+    do {                                                                                  \
+        if (a.f64_closure)                                                                \
+            hipLaunchKernelGGL((stage_kernel<K, I, true>), grid, block, 0, s, a);         \
+        else                                                                              \
+            hipLaunchKernelGGL((stage_kernel<K, I, false>), grid, block, 0, s, a);        \
+    } while (0)
     switch (kind) {
     case STAGE0:
         M1_LAUNCH(STAGE0, false);

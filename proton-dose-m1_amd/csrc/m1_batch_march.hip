@@ -360,8 +360,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         if (!c->have_problem)
             throw ValueError{"no problem bound: call m1_batch_set_problem first"};
         const Num n = read_numerics(num);
-        if (n.precision != M1_FP32_BULK)
-            throw NotImplemented{"the batched march runs the FP32-bulk kernels (M1_FP32_BULK)"};
+        if (!is_f32(n))
+            throw NotImplemented{"the batched march runs the FP32-bulk kernels (M1_FP32_BULK or "
+                                 "M1_FP32_BULK_F64_CLOSURE)"};
         bool ifac;
         if (n.scheme == M1_IF_SSPRK3)
             ifac = has_sources(n);
@@ -489,6 +490,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             a.coef = c->d_coef32;
             a.tile_max = skipping ? c->d_tile_max : nullptr;
             a.wave_mode = n.wave;
+            a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
             {
                 a.src = c->U;
                 a.out = c->V1;

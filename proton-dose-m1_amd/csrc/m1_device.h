@@ -131,6 +131,7 @@ struct StageArgs32 {
     // evaluate; STAGE2 writes the tile's max phi to tile_max[t].
     const uint8_t* compute;
     float* tile_max;
+    int f64_closure;        // face projection, closure, eigenvalues, HLL in double
 };
 
 // Workgroup tile geometry of the FP32 stage kernels (cells per tile in x, y, z).
@@ -215,6 +216,7 @@ struct BatchStageArgs {
     const StepCoef32* coef;  // per box
     float* tile_max;         // per global tile (nullable)
     int wave_mode;
+    int f64_closure;         // face projection, closure, eigenvalues, HLL in double
 };
 
 struct BatchCflArgs {

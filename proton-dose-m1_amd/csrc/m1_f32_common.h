@@ -1,6 +1,13 @@
 // FP32-bulk stage arithmetic shared by the single-box kernels
-// (m1_kernels_f32.hip) and the batched-beamlet kernels (m1_batch.hip).
+// (m1_kernels_f32.hip) and the batched-beamlet kernels (m1_batch_kernels.hip).
 // See m1_kernels_f32.hip for the precision split.
+//
+// These kernels are compiled with -ffp-contract=off: every fused multiply-add
+// is written out (fmaf), so that the four moments of a cell always go
+// through identical roundings.  A beam whose J equals phi along an axis then
+// keeps J = phi bit for bit; compiler-chosen fusion was found to differ
+// between the moments and to push such a beam off the realizability margin,
+// where the closure is far more diffusive across the beam.
 #pragma once
 
 #include "m1_device.h"
@@ -46,26 +53,26 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float weno_left(float v0, float v1, float v2, float v3, float v4,
                                            float eps)
 {
-    const float p0 = v0 * (1.f / 3.f) - v1 * (7.f / 6.f) + v2 * (11.f / 6.f);
-    const float p1 = v2 * (5.f / 6.f) - v1 * (1.f / 6.f) + v3 * (1.f / 3.f);
-    const float p2 = v2 * (1.f / 3.f) + v3 * (5.f / 6.f) - v4 * (1.f / 6.f);
-    float a = v0 - 2.f * v1 + v2;
-    float b = v0 - 4.f * v1 + 3.f * v2;
-    const float b0 = (13.f / 12.f) * (a * a) + 0.25f * (b * b);
-    a = v1 - 2.f * v2 + v3;
+    const float p0 = fmaf(v0, 1.f / 3.f, fmaf(v1, -7.f / 6.f, v2 * (11.f / 6.f)));
+    const float p1 = fmaf(v2, 5.f / 6.f, fmaf(v1, -1.f / 6.f, v3 * (1.f / 3.f)));
+    const float p2 = fmaf(v2, 1.f / 3.f, fmaf(v3, 5.f / 6.f, v4 * (-1.f / 6.f)));
+    float a = fmaf(-2.f, v1, v0) + v2;
+    float b = fmaf(3.f, v2, fmaf(-4.f, v1, v0));
+    const float b0 = fmaf((13.f / 12.f) * a, a, 0.25f * (b * b));
+    a = fmaf(-2.f, v2, v1) + v3;
     b = v1 - v3;
-    const float b1 = (13.f / 12.f) * (a * a) + 0.25f * (b * b);
-    a = v2 - 2.f * v3 + v4;
-    b = 3.f * v2 - 4.f * v3 + v4;
-    const float b2 = (13.f / 12.f) * (a * a) + 0.25f * (b * b);
+    const float b1 = fmaf((13.f / 12.f) * a, a, 0.25f * (b * b));
+    a = fmaf(-2.f, v3, v2) + v4;
+    b = fmaf(-4.f, v3, fmaf(3.f, v2, v4));
+    const float b2 = fmaf((13.f / 12.f) * a, a, 0.25f * (b * b));
     const float tau = fabsf(b2 - b0);
     const float r0 = fminf(tau * rcpf(b0 + eps), 1e18f);
     const float r1 = fminf(tau * rcpf(b1 + eps), 1e18f);
     const float r2 = fminf(tau * rcpf(b2 + eps), 1e18f);
-    const float a0 = 0.1f * (1.f + r0 * r0);
-    const float a1 = 0.6f * (1.f + r1 * r1);
-    const float a2 = 0.3f * (1.f + r2 * r2);
-    return (a0 * p0 + a1 * p1 + a2 * p2) * rcpf(a0 + a1 + a2);
+    const float a0 = fmaf(0.1f * r0, r0, 0.1f);
+    const float a1 = fmaf(0.6f * r1, r1, 0.6f);
+    const float a2 = fmaf(0.3f * r2, r2, 0.3f);
+    return fmaf(a0, p0, fmaf(a1, p1, a2 * p2)) * rcpf(a0 + a1 + a2);
 }
 
 // Realizability projection in double (realizability.py:132-176).
@@ -92,6 +99,157 @@ __device__ __forceinline__ void project_f(float u[4])
     u[1] = float(x);
     u[2] = float(y);
     u[3] = float(z);
+}
+
+// ---------------------------------------------------------------- float faces
+// The float face path keeps every operation in float except the one quantity
+// that loses its digits as |J|/phi -> 1: m = 1 - |J|^2/phi^2.  It enters the
+// closure through 1 - chi = 2 m / (1 + sqrt(1 + 3 m)) (the Levermore closure
+// with 4 - 3 f^2 = 1 + 3 m) and the eigenvalue discriminant, which the
+// identity 1 - f^2 = a + 3 a^2 / 4 (a = 1 - chi) turns into the sum of
+// non-negative terms 4 a (1 - mu^2) + 6 a^2 (1 + mu^2).  m itself is formed
+// from error-free products and sums (fma and TwoSum), so it is exact to a few
+// float ulps of m even where phi^2 - |J|^2 cancels.
+
+// 1 - (1 - 1e-10)^2: states beyond it are on the realizability margin of
+// realizability.py:182 (the reference scales them to |J| = (1 - 1e-10) phi).
+constexpr float M_MIN = 2.0e-10f;
+
+// phi^2 - |J|^2 for phi, J already scaled to phi in [0.5, 1).  The error
+// terms are exact because no product here is fused into a later sum
+// (-ffp-contract=off).
+__device__ __forceinline__ float deficit(float p, float x, float y, float z)
+{
+    const float p2 = p * p, p2e = __builtin_fmaf(p, p, -p2);
+    const float x2 = x * x, x2e = __builtin_fmaf(x, x, -x2);
+    const float y2 = y * y, y2e = __builtin_fmaf(y, y, -y2);
+    const float z2 = z * z, z2e = __builtin_fmaf(z, z, -z2);
+    const float t = x2 + y2;
+    float bp = t - x2;
+    const float te = (x2 - (t - bp)) + (y2 - bp);
+    const float u = t + z2;
+    bp = u - t;
+    const float ue = (t - (u - bp)) + (z2 - bp);
+    return (p2 - u) + ((((p2e - x2e) - y2e) - z2e) - (te + ue));
+}
+
+// Projected face state with its closure quantities (realizability.py:132-176,
+// flux.py:10-61): phi >= 0, |J| <= phi, m = 1 - f^2 >= M_MIN, a = 1 - chi.
+// No reciprocal of phi or |J| is formed, so tiny and subnormal states stay
+// finite.
+struct FaceF {
+    float p, J[3], n[3], jn, f, a;  // n = J / |J| (0 for J = 0)
+};
+
+__device__ __forceinline__ FaceF face_state(float q0, float q1, float q2, float q3)
+{
+    FaceF c;
+    const float p = fmaxf(q0, 0.f);
+    c.p = p;
+    if (!(p > 0.f)) {
+        // vacuum: J is projected to 0, f = 0, chi = 1/3
+        c.J[0] = c.J[1] = c.J[2] = 0.f;
+        c.n[0] = c.n[1] = c.n[2] = 0.f;
+        c.jn = c.f = 0.f;
+        c.a = 2.f / 3.f;
+        return c;
+    }
+    // Exact power-of-two scaling by the largest magnitude keeps every square
+    // below 1 (and phi in [0.5, 1) whenever |J| < phi).
+    const float big = fmaxf(fmaxf(p, fabsf(q1)), fmaxf(fabsf(q2), fabsf(q3)));
+    const int e = __builtin_amdgcn_frexp_expf(big);
+    const float ps = __builtin_ldexpf(p, -e), xs = __builtin_ldexpf(q1, -e),
+                ys = __builtin_ldexpf(q2, -e), zs = __builtin_ldexpf(q3, -e);
+    const float d = deficit(ps, xs, ys, zs);
+    // Direction from the dominant component k: with t = J / J_k,
+    // n = t / |t| and |t|^2 = 1 + (the two ratios squared).  Where the other
+    // two components are below ~2.4e-4 of J_k, |t|^2 rounds to 1 and n_k is
+    // +-1 exactly, so a beam with J = phi along an axis keeps J = phi bit for
+    // bit through fluxes and projections.
+    const float ax = fabsf(xs), ay = fabsf(ys), az = fabsf(zs);
+    const bool kz = az >= ax && az >= ay;
+    const bool ky = !kz && ay >= ax;
+    const float jk = kz ? zs : (ky ? ys : xs);
+    const float u1 = kz ? xs : (ky ? xs : ys);  // the two other components,
+    const float u2 = kz ? ys : (ky ? zs : zs);  // in axis order
+    const bool nz = jk != 0.f;
+    // |u| <= |J_k|: a second exact scaling by J_k's exponent keeps the
+    // reciprocal finite however small J is
+    const int ek = nz ? __builtin_amdgcn_frexp_expf(jk) : 0;
+    const float ik = nz ? rcpf(__builtin_ldexpf(jk, -ek)) : 0.f;
+    const float t1 = __builtin_ldexpf(u1, -ek) * ik, t2 = __builtin_ldexpf(u2, -ek) * ik;
+    const float tt = 1.f + (t1 * t1 + t2 * t2);
+    const float r = tt == 1.f ? 1.f : __builtin_amdgcn_rsqf(tt);
+    const float nk = (jk > 0.f ? 1.f : -1.f) * r;
+    const float n1 = t1 * nk, n2 = t2 * nk;  // t carries the sign of J_k
+    c.n[0] = kz ? n1 : (ky ? n1 : nk);
+    c.n[1] = kz ? n2 : (ky ? nk : n1);
+    c.n[2] = kz ? nk : n2;
+    if (!nz)
+        c.n[0] = c.n[1] = c.n[2] = 0.f;
+    const float jns = nz ? fabsf(jk) * (tt == 1.f ? 1.f : tt * r) : 0.f;
+    if (d < M_MIN * (ps * ps)) {
+        // on or beyond the margin: |J| scaled to phi, f = 1 - 1e-10 to float
+        c.J[0] = p * c.n[0];
+        c.J[1] = p * c.n[1];
+        c.J[2] = p * c.n[2];
+        c.jn = p;
+        c.f = 1.f;
+        c.a = 2.f * M_MIN * rcpf(1.f + __builtin_amdgcn_sqrtf(1.f + 3.f * M_MIN));
+        return c;
+    }
+    // here |J| < phi, so big = phi and ps lies in [0.5, 1)
+    const float inv_ps = rcpf(ps);
+    const float m = d * (inv_ps * inv_ps);
+    c.J[0] = q1;
+    c.J[1] = q2;
+    c.J[2] = q3;
+    c.jn = __builtin_ldexpf(jns, e);
+    c.f = fminf(jns * inv_ps, 1.f);
+    c.a = 2.f * m * rcpf(1.f + __builtin_amdgcn_sqrtf(1.f + 3.f * m));
+    return c;
+}
+
+// Stage-state projection in float (realizability.py:132-176).
+__device__ __forceinline__ void project_ff(float u[4])
+{
+    const FaceF c = face_state(u[0], u[1], u[2], u[3]);
+    u[0] = c.p;
+    u[1] = c.J[0];
+    u[2] = c.J[1];
+    u[3] = c.J[2];
+}
+
+// Extreme eigenvalues along AXIS (wave_speeds.py:37-103) from the closure:
+// disc = 4 a (1 - mu^2) + 6 a^2 (1 + mu^2), lambda = (2 f mu -+ sqrt(disc)) / (2 + 3 a).
+template <int AXIS>
+__device__ __forceinline__ void eigen_f(const FaceF& c, float eps, float& lmin, float& lmax)
+{
+    if (c.jn < eps) {
+        lmin = -0.57735026918962576f;
+        lmax = 0.57735026918962576f;
+        return;
+    }
+    const float n0 = c.n[AXIS];
+    const float n1 = c.n[(AXIS + 1) % 3];
+    const float n2 = c.n[(AXIS + 2) % 3];
+    const float mu2 = n0 * n0;
+    const float disc = 4.f * c.a * (n1 * n1 + n2 * n2) + 6.f * c.a * c.a * (1.f + mu2);
+    const float root = __builtin_amdgcn_sqrtf(disc);
+    const float inv = rcpf(2.f + 3.f * c.a);
+    lmin = (2.f * c.f * n0 - root) * inv;
+    lmax = (2.f * c.f * n0 + root) * inv;
+}
+
+template <int AXIS>
+__device__ __forceinline__ void phys_flux_f(const FaceF& c, float F[4])
+{
+    const float iso = 0.5f * c.a, dir = 1.f - 1.5f * c.a;
+    const float na = c.n[AXIS];
+    F[0] = c.J[AXIS];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        F[1 + k] = c.p * (dir * na * c.n[k] + (k == AXIS ? iso : 0.f));
 }
 
 struct Closure {
@@ -155,51 +313,91 @@ __device__ __forceinline__ void phys_flux(double phi, const double J[3], const C
         F[1 + k] = phi * (c.dir * c.n[AXIS] * c.n[k] + (k == AXIS ? c.iso : 0.0));
 }
 
-// Face flux from six cell states s[c][0..5] (cells f-3..f+2 of face f-1/2).
+// HLL flux of two projected float face states (flux.py:358-384).
 template <int AXIS>
+__device__ __forceinline__ void hll_f(const FaceF& cl, const FaceF& cr, int wave, float eps,
+                                      float F[4])
+{
+    float FL[4], FR[4];
+    phys_flux_f<AXIS>(cl, FL);
+    phys_flux_f<AXIS>(cr, FR);
+    float sl = -1.f, sr = 1.f;
+    if (wave == 0) {
+        float a, b, c, d;
+        eigen_f<AXIS>(cl, eps, a, b);
+        eigen_f<AXIS>(cr, eps, c, d);
+        sl = fminf(fminf(a, c), 0.f);
+        sr = fmaxf(fmaxf(b, d), 0.f);
+    }
+    const float inv = rcpf(sr - sl);
+    const float sp = sl * sr;
+    const float Lp[4] = {cl.p, cl.J[0], cl.J[1], cl.J[2]};
+    const float Rp[4] = {cr.p, cr.J[0], cr.J[1], cr.J[2]};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        F[c] = (sr * FL[c] - sl * FR[c] + sp * (Rp[c] - Lp[c])) * inv;
+}
+
+// Face flux from six cell states s[c][0..5] (cells f-3..f+2 of face f-1/2).
+// D: projection, closure, eigenvalues and HLL in double; otherwise in float
+// with the cancellation-free closure above.
+template <int AXIS, bool D>
 __device__ __forceinline__ void face_flux(const float s[4][6], int wave, float weps, double eeps,
                                           float F[4])
 {
-    double L[4], R[4];
+    if constexpr (D) {
+        double L[4], R[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        L[c] = weno_left(s[c][0], s[c][1], s[c][2], s[c][3], s[c][4], weps);
-        R[c] = weno_left(s[c][5], s[c][4], s[c][3], s[c][2], s[c][1], weps);
-    }
-    double jl, jr;
-    project(L[0], L[1], L[2], L[3], jl);
-    project(R[0], R[1], R[2], R[3], jr);
-    const Closure cl = closure(L[0], L + 1, jl);
-    const Closure cr = closure(R[0], R + 1, jr);
-    double FL[4], FR[4];
-    phys_flux<AXIS>(L[0], L + 1, cl, FL);
-    phys_flux<AXIS>(R[0], R + 1, cr, FR);
-    double sl = -1.0, sr = 1.0;
-    if (wave == 0) {
-        double a, b, c, d;
-        eigen<AXIS>(L[0], cl, eeps, a, b);
-        eigen<AXIS>(R[0], cr, eeps, c, d);
-        sl = fmin(fmin(a, c), 0.0);
-        sr = fmax(fmax(b, d), 0.0);
-    }
-    const double inv = 1.0 / (sr - sl);
-    const double sp = sl * sr;
+        for (int c = 0; c < 4; ++c) {
+            L[c] = weno_left(s[c][0], s[c][1], s[c][2], s[c][3], s[c][4], weps);
+            R[c] = weno_left(s[c][5], s[c][4], s[c][3], s[c][2], s[c][1], weps);
+        }
+        double jl, jr;
+        project(L[0], L[1], L[2], L[3], jl);
+        project(R[0], R[1], R[2], R[3], jr);
+        const Closure cl = closure(L[0], L + 1, jl);
+        const Closure cr = closure(R[0], R + 1, jr);
+        double FL[4], FR[4];
+        phys_flux<AXIS>(L[0], L + 1, cl, FL);
+        phys_flux<AXIS>(R[0], R + 1, cr, FR);
+        double sl = -1.0, sr = 1.0;
+        if (wave == 0) {
+            double a, b, c, d;
+            eigen<AXIS>(L[0], cl, eeps, a, b);
+            eigen<AXIS>(R[0], cr, eeps, c, d);
+            sl = fmin(fmin(a, c), 0.0);
+            sr = fmax(fmax(b, d), 0.0);
+        }
+        const double inv = 1.0 / (sr - sl);
+        const double sp = sl * sr;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        F[c] = float((sr * FL[c] - sl * FR[c] + sp * (R[c] - L[c])) * inv);
+        for (int c = 0; c < 4; ++c)
+            F[c] = float((sr * FL[c] - sl * FR[c] + sp * (R[c] - L[c])) * inv);
+    } else {
+        float L[4], R[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            L[c] = weno_left(s[c][0], s[c][1], s[c][2], s[c][3], s[c][4], weps);
+            R[c] = weno_left(s[c][5], s[c][4], s[c][3], s[c][2], s[c][1], weps);
+        }
+        const FaceF cl = face_state(L[0], L[1], L[2], L[3]);
+        const FaceF cr = face_state(R[0], R[1], R[2], R[3]);
+        hll_f<AXIS>(cl, cr, wave, float(eeps), F);
+    }
 }
 
+template <bool D>
 __device__ __forceinline__ void face_flux_yz(bool zaxis, const float s[4][6], int wave,
                                              float weps, double eeps, float F[4])
 {
     if (zaxis)
-        face_flux<2>(s, wave, weps, eeps, F);
+        face_flux<2, D>(s, wave, weps, eeps, F);
     else
-        face_flux<1>(s, wave, weps, eeps, F);
+        face_flux<1, D>(s, wave, weps, eeps, F);
 }
 
 // Stencil input of one cell: U directly, or proj(C_k V_k / S_k) for stages 1, 2.
-template <int KIND>
+template <int KIND, bool D>
 __device__ __forceinline__ void load_state(const TileView& a, const StepCoef32& c, long i,
                                            long N, float u[4])
 {
@@ -215,7 +413,10 @@ __device__ __forceinline__ void load_state(const TileView& a, const StepCoef32& 
         u[1] *= gj;
         u[2] *= gj;
         u[3] *= gj;
-        project_f(u);
+        if constexpr (D)
+            project_f(u);
+        else
+            project_ff(u);
     }
 }
 
@@ -370,7 +571,7 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
     lo += err;
 }
 
-template <int KIND, bool IFAC>
+template <int KIND, bool IFAC, bool D>
 __device__ __forceinline__ void stage_tile(const TileView& a)
 {
     __shared__ StepCoef32 sc;
@@ -423,7 +624,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     float W[6][4];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-        load_state<KIND>(a, sc, cidx(clampx(x0 - 3 + k), yc, zc), N, W[k]);
+        load_state<KIND, D>(a, sc, cidx(clampx(x0 - 3 + k), yc, zc), N, W[k]);
     float Flo[4];
     {
         float s[4][6];
@@ -432,7 +633,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
             for (int k = 0; k < 6; ++k)
                 s[c][k] = W[k][c];
-        face_flux<0>(s, wave, weps, eeps, Flo);
+        face_flux<0, D>(s, wave, weps, eeps, Flo);
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k)
@@ -441,7 +642,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
             W[k][c] = W[k + 1][c];
 
     for (int x = x0; x < x1; ++x) {
-        load_state<KIND>(a, sc, cidx(clampx(x + 3), yc, zc), N, W[5]);
+        load_state<KIND, D>(a, sc, cidx(clampx(x + 3), yc, zc), N, W[5]);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
             P[c][ty + 3][tz + 3] = W[2][c];
@@ -451,7 +652,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 float q[4][4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    load_state<KIND>(a, sc, cidx(x, hgy, k), N, q[k]);
+                    load_state<KIND, D>(a, sc, cidx(x, hgy, k), N, q[k]);
                 const long b = long(x) * ny + hgy;
                 const long m2 = long(nx) * ny;
 #pragma unroll
@@ -459,7 +660,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     u[c] = zmin_ghost(-hgz_raw, a.bnd[c * m2 + b], q[0][c], q[1][c], q[2][c],
                                       q[3][c]);
             } else {
-                load_state<KIND>(a, sc, cidx(x, hgy, hgz), N, u);
+                load_state<KIND, D>(a, sc, cidx(x, hgy, hgz), N, u);
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -473,7 +674,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
                     s[c][k] = W[k][c];
-            face_flux<0>(s, wave, weps, eeps, Fhi);
+            face_flux<0, D>(s, wave, weps, eeps, Fhi);
         }
         __syncthreads();
         {
@@ -483,7 +684,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
                     s[c][k] = P[c][ty + 3][tz + k];
-            face_flux<2>(s, wave, weps, eeps, F);
+            face_flux<2, D>(s, wave, weps, eeps, F);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 Fz[c][ty][tz] = F[c];
@@ -492,7 +693,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
                     s[c][k] = P[c][ty + k][tz + 3];
-            face_flux<1>(s, wave, weps, eeps, F);
+            face_flux<1, D>(s, wave, weps, eeps, F);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 Fy[c][ty][tz] = F[c];
@@ -507,7 +708,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
                     s[c][k] = zaxis ? P[c][row + 3][TZ + k] : P[c][TY + k][col + 3];
-            face_flux_yz(zaxis, s, wave, weps, eeps, F);
+            face_flux_yz<D>(zaxis, s, wave, weps, eeps, F);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 if (zaxis)
@@ -525,8 +726,9 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
             float rhs[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                rhs[c] = -((Fhi[c] - Flo[c]) * idx + (Fy[c][ty + 1][tz] - Fy[c][ty][tz]) * idy +
-                           (Fz[c][ty][tz + 1] - Fz[c][ty][tz]) * idz);
+                rhs[c] = -fmaf(Fhi[c] - Flo[c], idx,
+                               fmaf(Fy[c][ty + 1][tz] - Fy[c][ty][tz], idy,
+                                    (Fz[c][ty][tz + 1] - Fz[c][ty][tz]) * idz));
             if constexpr (KIND == STAGE_RHS) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
@@ -535,14 +737,17 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 const float S0 = sc.S0[m];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = rhs[c] * dE + S0 * W[2][c];
+                    a.out[i + c * N] = fmaf(rhs[c], dE, S0 * W[2][c]);
             } else if constexpr (KIND == STAGE_FE) {
                 const float S0 = sc.S0[m];
                 float u[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    u[c] = (rhs[c] * dE + S0 * W[2][c]) * (1.f / sc.S1[m]);
-                project_f(u);
+                    u[c] = fmaf(rhs[c], dE, S0 * W[2][c]) * (1.f / sc.S1[m]);
+                if constexpr (D)
+                    project_f(u);
+                else
+                    project_ff(u);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     a.out[i + c * N] = u[c];
@@ -558,7 +763,8 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     float r = rhs[c];
                     if constexpr (IFAC)
                         r *= (c == 0 ? sc.i1p[m] : sc.i1j[m]);
-                    a.out[i + c * N] = 0.75f * (S0 * a.U[i + c * N]) + 0.25f * (r * dE + a.V1[i + c * N]);
+                    a.out[i + c * N] =
+                        fmaf(0.75f, S0 * a.U[i + c * N], 0.25f * fmaf(r, dE, a.V1[i + c * N]));
                 }
             } else {  // STAGE2
                 const float S0 = sc.S0[m], S2 = sc.S2[m];
@@ -571,10 +777,14 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     float r = rhs[c];
                     if constexpr (IFAC)
                         r *= (c == 0 ? sc.i2p[m] : sc.i2j[m]);
-                    const float v = (1.f / 3.f) * (S0 * un[c]) + (2.f / 3.f) * (r * dE + a.src[i + c * N]);
+                    const float v = fmaf(1.f / 3.f, S0 * un[c],
+                                         (2.f / 3.f) * fmaf(r, dE, a.src[i + c * N]));
                     u[c] = v * (c == 0 ? sc.g1p[m] : sc.g1j[m]);
                 }
-                project_f(u);
+                if constexpr (D)
+                    project_f(u);
+                else
+                    project_ff(u);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     a.U[i + c * N] = u[c];
@@ -582,7 +792,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 const float q0 = un[0] * S0;
                 const float q1 = fmaxf(a.V1[i] * sc.g1p[m], 0.f) * sc.S1[m];
                 const float q2 = W[2][0] * S2;
-                const float e = dE * ((1.f / 6.f) * (q0 + q1) + (2.f / 3.f) * q2);
+                const float e = dE * fmaf(1.f / 6.f, q0 + q1, (2.f / 3.f) * q2);
                 if (a.edep)
                     a.edep[i] = e;
                 if (a.dose_hi)

@@ -99,7 +99,7 @@ inline Num read_numerics(const m1_numerics* n)
         throw ValueError{"Unknown scattering model."};
     if (r.interp != M1_INTERP_LINEAR && r.interp != M1_INTERP_PCHIP)
         throw ValueError{"Unknown coefficient interpolation."};
-    if (r.precision != M1_FP64 && r.precision != M1_FP64_FAST && r.precision != M1_FP32_BULK)
+    if (r.precision < M1_FP64 || r.precision > M1_FP32_BULK_F64_CLOSURE)
         throw ValueError{"Unknown precision."};
     if (!(std::isfinite(r.max_step) && std::isfinite(r.cfl) && std::isfinite(r.emin) &&
           std::isfinite(r.window)))
@@ -116,6 +116,12 @@ inline Num read_numerics(const m1_numerics* n)
 }
 
 inline bool has_sources(const Num& n) { return n.scattering != M1_SCATTERING_OFF || n.nuclear; }
+
+// The march runs the float-state kernels.
+inline bool is_f32(const Num& n)
+{
+    return n.precision == M1_FP32_BULK || n.precision == M1_FP32_BULK_F64_CLOSURE;
+}
 
 // Material tables of the ABI, checked as stopping_power.py:73-91 checks them.
 inline MaterialData read_material(const m1_material& m, int idx)

@@ -25,7 +25,7 @@ namespace {
 
 using namespace detail;
 
-template <int KIND, bool IFAC>
+template <int KIND, bool IFAC, bool D>
 __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
 {
     const long tile = (long(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
     TileView v{a.g,      a.src,       a.U,   a.V1,  a.out,  a.dose_hi,
                a.dose_lo, a.edep,     a.ids, a.bnd, a.coef, a.tile_max ? a.tile_max + tile : nullptr,
                a.wave_mode, int(blockIdx.z), int(blockIdx.y), int(blockIdx.x)};
-    stage_tile<KIND, IFAC>(v);
+    stage_tile<KIND, IFAC, D>(v);
 }
 
 // ---------------------------------------------------------------- CFL
@@ -196,7 +196,13 @@ hipError_t launch_stage(int kind, bool ifac, const StageArgs32& a, hipStream_t s
 {
     const dim3 grid((a.g.nz + TZ - 1) / TZ, (a.g.ny + TY - 1) / TY, (a.g.nx + TX - 1) / TX);
     const dim3 block(NT);
-#define M1_LAUNCH(K, I) hipLaunchKernelGGL((stage_kernel<K, I>), grid, block, 0, s, a)
+#define M1_LAUNCH(K, I)                                                                   \
+    do {                                                                                  \
+        if (a.f64_closure)                                                                \
+            hipLaunchKernelGGL((stage_kernel<K, I, true>), grid, block, 0, s, a);         \
+        else                                                                              \
+            hipLaunchKernelGGL((stage_kernel<K, I, false>), grid, block, 0, s, a);        \
+    } while (0)
     switch (kind) {
     case STAGE0:
         M1_LAUNCH(STAGE0, false);

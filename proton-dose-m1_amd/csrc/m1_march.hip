@@ -375,6 +375,7 @@ m1::StageArgs32 base_args32(const m1_ctx* c, const Num& n)
     a.ids = c->ids;
     a.coef = c->d_step32;
     a.wave_mode = n.wave;
+    a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
     return a;
 }
 
@@ -783,7 +784,7 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
         };
         record(energy);
         size_t ti = 0;
-        const bool f32 = n.precision == M1_FP32_BULK;
+        const bool f32 = m1::is_f32(n);
         if (f32) {
             narrow_state(ctx);
             hip_check(hipMemsetAsync(ctx->dose_hi, 0, size_t(ctx->N) * sizeof(float), ctx->stream),
@@ -859,7 +860,7 @@ int m1_spatial_operator(m1_ctx* ctx, const m1_numerics* num, double energy,
         if (!rhs)
             throw ValueError{"rhs must not be NULL"};
         const size_t m2 = size_t(ctx->nx) * ctx->ny;
-        const bool f32 = n.precision == M1_FP32_BULK;
+        const bool f32 = m1::is_f32(n);
         if (boundary_state)
             hip_check(hipMemcpyAsync(ctx->bnd, boundary_state, 4 * m2 * sizeof(double),
                                      hipMemcpyHostToDevice, ctx->stream),
@@ -914,7 +915,7 @@ int m1_compute_energy_step(m1_ctx* ctx, const m1_numerics* num, double energy, d
         require_problem(ctx);
         const Num n = read_numerics(num);
         double d;
-        if (n.precision == M1_FP32_BULK) {
+        if (m1::is_f32(n)) {
             narrow_state(ctx);
             d = cfl_step_f32(ctx, n, energy);
         } else {
@@ -931,7 +932,7 @@ int m1_step(m1_ctx* ctx, const m1_numerics* num, int32_t kind, double energy, do
     return guarded(ctx, [&] {
         require_problem(ctx);
         const Num n = read_numerics(num);
-        if (n.precision == M1_FP32_BULK) {
+        if (m1::is_f32(n)) {
             narrow_state(ctx);
             step_f32(ctx, n, kind, energy, dE, false, edep != nullptr);
             widen_state(ctx, ctx->U);

@@ -144,10 +144,12 @@ _SCHEME = {EnergyScheme.FORWARD_EULER: 0, EnergyScheme.SSPRK3: 1, EnergyScheme.I
 _WAVE = {WaveSpeeds.EXACT: 0, WaveSpeeds.UNIT: 1}
 _SCAT = {Scattering.OFF: 0, Scattering.ROSSI: 1, Scattering.WENTZEL: 2}
 _INTERP = {CoefficientInterpolation.LINEAR: 0, CoefficientInterpolation.PCHIP: 1}
-PRECISION_CODE = {Precision.FP64: 0, Precision.FP32_BULK: 1, Precision.FP64_FAST: 2}
+PRECISION_CODE = {Precision.FP64: 0, Precision.FP32_BULK: 1, Precision.FP64_FAST: 2,
+                  Precision.FP32_BULK_F64_CLOSURE: 3}
 FP64 = 0
 FP32_BULK = 1
 FP64_FAST = 2
+FP32_BULK_F64_CLOSURE = 3
 
 
 def numerics_struct(config, precision_code: int | None = None) -> m1_numerics:

@@ -272,7 +272,7 @@ def batch_march(phantom: Phantom, beamlets, config, boxes=None, want_dose=True,
 
     `config` supplies physics and numerics (a SolverConfig; its geometry and
     beam are not used).  The batch runs the FP32-bulk march."""
-    if config.device.precision is not Precision.FP32_BULK:
+    if config.device.precision not in (Precision.FP32_BULK, Precision.FP32_BULK_F64_CLOSURE):
         raise NotImplementedError("the batched march runs the FP32-bulk kernels "
                                   "(DeviceOptions(precision=Precision.FP32_BULK))")
     if handle is None:

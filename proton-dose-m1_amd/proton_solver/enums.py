@@ -70,11 +70,16 @@ class Precision(Enum):
     FP64 reproduces the reference's all-double march with its operation
     order, every operation separately rounded.  FP64_FAST is the same march
     with fused multiply-adds and reciprocal constants (roundoff-level
-    differences).  FP32_BULK runs reconstruction, flux and stage algebra in
-    float and keeps the closure, the realizability projection, the
-    integrating factors and a compensated dose accumulation in double.
+    differences).  FP32_BULK keeps the state and runs reconstruction,
+    projection, closure, eigenvalues, HLL fluxes and stage algebra in float,
+    forms the one cancelling quantity 1 - |J|^2/phi^2 exactly (error-free
+    float products and sums), and keeps the step scalars and integrating
+    factors, the inflow extrapolation and a compensated dose accumulation in
+    double or float pairs.  FP32_BULK_F64_CLOSURE is FP32_BULK with the face
+    projection, closure, eigenvalues and HLL combination in double.
     """
 
     FP64 = auto()
     FP32_BULK = auto()
     FP64_FAST = auto()
+    FP32_BULK_F64_CLOSURE = auto()

@@ -21,21 +21,30 @@ from proton_solver import Box, Material, Physics, Scattering
 pytestmark = pytest.mark.gpu
 
 F32 = _native.FP32_BULK
+# FP32_BULK (float closure with the exact 1 - f^2) and FP32_BULK_F64_CLOSURE
+# (face projection, closure, eigenvalues and HLL in double)
+VARIANTS = {"float_closure": _native.FP32_BULK, "f64_closure": _native.FP32_BULK_F64_CLOSURE}
 MAX_DOSE_TOL = 5e-3
 GAMMA_MIN = 99.5
 
 
-def test_fp32_spatial_operator_close_to_oracle():
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_fp32_spatial_operator_close_to_oracle(variant):
     cfg = small_config(shape=(11, 13, 37), objects=HETERO)
     grid, mm, inflow, ctx, orc = setup(cfg)
-    num = _native.numerics_struct(cfg, F32)
+    num = _native.numerics_struct(cfg, VARIANTS[variant])
     for seed in (1, 2):
         U = random_state(grid.shape, seed)
         ctx.set_state(U)
         rhs = ctx.spatial_operator(num, 101.3)
         ref = orc.spatial_operator(U, 101.3)
+        bad = ~np.isfinite(rhs)
+        if bad.any():
+            c, i, j, k = (int(v[0]) for v in np.nonzero(bad))
+            say("non-finite rhs at", (c, i, j, k), "count", int(bad.sum()), "U there",
+                U[:, i, j, k].tolist())
         err = np.abs(rhs - ref).max() / np.abs(ref).max()
-        say("fp32 spatial operator seed", seed, "max|d|/max", err)
+        say("fp32", variant, "spatial operator seed", seed, "max|d|/max", err)
         # float WENO on rough random data: the nonlinear weights at the many
         # jumps react to float rounding of the smoothness indicators (1.3e-4
         # observed); smooth march data stays ~1e-5 (see the march tests)
@@ -54,13 +63,14 @@ def test_fp32_cfl_close_to_oracle():
     assert got == pytest.approx(ref, rel=1e-5)
 
 
+@pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("kind", list(STEP_KINDS))
-def test_fp32_single_step_close_to_oracle(kind):
+def test_fp32_single_step_close_to_oracle(kind, variant):
     bone = Box(-0.004, 0.0, -0.005, 0.005, 0.004, 0.010, Material.BONE_COMPACT)
     cfg = small_config(shape=(9, 10, 20), physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
                        objects=(bone,))
     grid, mm, inflow, ctx, orc = setup(cfg)
-    num = _native.numerics_struct(cfg, F32)
+    num = _native.numerics_struct(cfg, VARIANTS[variant])
     U = random_state(grid.shape, 7, scale=1e8)
     ctx.set_state(U)
     edep = ctx.step(num, STEP_KINDS[kind], 99.0, 0.37)
@@ -68,13 +78,13 @@ def test_fp32_single_step_close_to_oracle(kind):
     U_ref, edep_ref = orc.step(U, 99.0, 0.37, kind)
     eu = np.abs(U_gpu - U_ref).max() / np.abs(U_ref).max()
     ee = np.abs(edep - edep_ref).max() / np.abs(edep_ref).max()
-    say("fp32 step", kind, "state err", eu, "edep err", ee)
+    say("fp32", variant, "step", kind, "state err", eu, "edep err", ee)
     assert eu < 1e-4 and ee < 1e-4
 
 
-def _accuracy(cfg, label):
+def _accuracy(cfg, label, precision=F32):
     grid, mm, inflow, ctx, orc = setup(cfg)
-    num = _native.numerics_struct(cfg, F32)
+    num = _native.numerics_struct(cfg, precision)
     E, _ = ctx.march(num)
     dose = ctx.get_dose()
     U = ctx.get_state()
@@ -115,16 +125,19 @@ def test_fp32_tile_skipping_matches_full_domain():
     assert err < 5e-4
 
 
-def test_fp32_config1_march_accuracy():
-    md, mx, gam, _ = _accuracy(CONFIG1, "config1 fp32")
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_fp32_config1_march_accuracy(variant):
+    md, mx, gam, _ = _accuracy(CONFIG1, "config1 fp32 " + variant, VARIANTS[variant])
     assert abs(md) <= MAX_DOSE_TOL
     assert gam >= GAMMA_MIN
 
 
+@pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("idx", range(6))
-def test_fp32_golden_cases_accuracy(idx):
+def test_fp32_golden_cases_accuracy(idx, variant):
     case = [c for c in thesis_cases() if "coarse_grid_2mm" in c["source"]][idx]
-    md, mx, gam, dose = _accuracy(config_from_fixture(case["config"]), case["source"])
+    md, mx, gam, dose = _accuracy(config_from_fixture(case["config"]),
+                                  case["source"] + " " + variant, VARIANTS[variant])
     assert abs(md) <= MAX_DOSE_TOL
     assert gam >= GAMMA_MIN
     assert abs(dose.max() / case["maximum_dose_Gy"] - 1) <= MAX_DOSE_TOL
