@@ -106,7 +106,7 @@ typedef struct {
     double cfl;
     double energy_min;                  /* MeV */
     double energy_window_sigma;
-    /* FP32_BULK modes only: workgroup tiles (16 x 8 x 32 cells) whose 9-cell
+    /* FP32_BULK modes only: workgroup tiles (16 x 16 x 16 cells) whose 9-cell
      * neighbourhood holds phi below tile_threshold x (peak inflow fluence)
      * are not evaluated and are kept as exact vacuum; 0 evaluates every
      * tile.  No reference counterpart (the reference's closest analogue is

@@ -37,6 +37,10 @@ COMMON = [
 
 
 def _units():
+    waves = [f"-DM1_STAGE_WAVES={int(os.environ['M1_STAGE_WAVES'])}"] \
+        if os.environ.get("M1_STAGE_WAVES") else []
+    if os.environ.get("M1_TILE_TY"):
+        waves.append(f"-DM1_TILE_TY={int(os.environ['M1_TILE_TY'])}")
     return [
         (CSRC / "m1_kernels.hip", BUILD / "m1_kernels_exact.o",
          ["-DM1_EXACT=1", "-ffp-contract=off"]),
@@ -44,10 +48,11 @@ def _units():
          ["-DM1_EXACT=0", "-ffp-contract=fast"]),
         # float kernels: fused multiply-adds only where written (fmaf), so the
         # four moments of a cell round alike (m1_f32_common.h)
-        (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=off"]),
-        (CSRC / "m1_march.hip", BUILD / "m1_march.o", ["-ffp-contract=off"]),
-        (CSRC / "m1_batch_kernels.hip", BUILD / "m1_batch_kernels.o", ["-ffp-contract=off"]),
-        (CSRC / "m1_batch_march.hip", BUILD / "m1_batch_march.o", ["-ffp-contract=off"]),
+        (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=off", *waves]),
+        (CSRC / "m1_march.hip", BUILD / "m1_march.o", ["-ffp-contract=off", *waves]),
+        (CSRC / "m1_batch_kernels.hip", BUILD / "m1_batch_kernels.o",
+         ["-ffp-contract=off", *waves]),
+        (CSRC / "m1_batch_march.hip", BUILD / "m1_batch_march.o", ["-ffp-contract=off", *waves]),
     ]
 
 

@@ -39,7 +39,7 @@ __device__ __forceinline__ TileCoord tile_coord(const BoxDesc& d, int b, int t)
 }
 
 template <int KIND, bool IFAC, bool D>
-__global__ void __launch_bounds__(NT) stage_kernel(BatchStageArgs a)
+__global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArgs a)
 {
     const int t = a.list[xcd_slot(int(blockIdx.x), a.n_list)];
     const int b = a.tile_box[t];

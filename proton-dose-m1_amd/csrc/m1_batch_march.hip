@@ -535,7 +535,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             {
                 BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
                                  int(T),        c->d_tile_max,   c->d_flags[cur],
-                                 c->d_flags[1 - cur], float(n.tile_threshold), 1, 2, 1,
+                                 c->d_flags[1 - cur], float(n.tile_threshold),
+                                 tile_radius(F32_TX), tile_radius(F32_TY), tile_radius(F32_TZ),
                                  skipping ? 0 : 1};
                 Timed t(c->prof, s, K_TILE_MASK);
                 hip_check(batch::launch_mask(ma, s), "tile mask kernel (batch)");

@@ -135,7 +135,15 @@ struct StageArgs32 {
 };
 
 // Workgroup tile geometry of the FP32 stage kernels (cells per tile in x, y, z).
-constexpr int F32_TX = 16, F32_TY = 8, F32_TZ = 32;
+#ifndef M1_TILE_TY
+#define M1_TILE_TY 16
+#endif
+constexpr int F32_TX = 16, F32_TY = M1_TILE_TY, F32_TZ = 256 / M1_TILE_TY;
+// Tile-skip dilation: a step's three stages reach 3 x 3 = 9 cells (the WENO-Z
+// stencil half-width per stage), so tiles within 9 cells of a tile holding
+// phi above the threshold are evaluated.
+constexpr int TILE_REACH = 9;
+constexpr int tile_radius(int t) { return (TILE_REACH + t - 1) / t; }
 
 struct TileMaskArgs {
     int ntx, nty, ntz;       // tiles per axis

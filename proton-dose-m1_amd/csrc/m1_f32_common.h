@@ -34,11 +34,20 @@ struct TileView {
     int tx, ty, tz;    // tile coordinates (x: TX planes, y: TY rows, z: TZ cells)
 };
 
+// Waves per SIMD the stage kernels are compiled for (register budget:
+// 512 / waves VGPRs per lane).
+#ifndef M1_STAGE_WAVES
+#define M1_STAGE_WAVES 3
+#endif
+
 namespace detail {
 
 constexpr int TY = F32_TY;
 constexpr int TZ = F32_TZ;
 constexpr int TX = F32_TX;
+static_assert(TY * TZ == 256, "one thread per y-z cell of a tile");
+static_assert(TX * TY <= 256, "the batch CFL kernel maps one thread to each z-min column");
+static_assert(TY + TZ <= 256, "one thread per upper tile face");
 constexpr int NT = TY * TZ;
 constexpr int PY = TY + 6;
 constexpr int PZ = TZ + 6;

@@ -26,7 +26,7 @@ namespace {
 using namespace detail;
 
 template <int KIND, bool IFAC, bool D>
-__global__ void __launch_bounds__(NT) stage_kernel(StageArgs32 a)
+__global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(StageArgs32 a)
 {
     const long tile = (long(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     if (a.compute && !a.compute[tile])

@@ -541,7 +541,8 @@ void update_tiles(m1_ctx* c, const Num& n, double energy_next)
     hip_check(hipMemsetAsync(c->d_nactive, 0, sizeof(int), c->stream), "tile count reset");
     m1::TileMaskArgs ma{c->ntx, c->nty, c->ntz, c->d_tile_max, c->d_compute[c->cur],
                         c->d_compute[1 - c->cur], float(n.tile_threshold), inflow_on,
-                        1, 2, 1, c->d_nactive};
+                        m1::tile_radius(m1::F32_TX), m1::tile_radius(m1::F32_TY),
+                        m1::tile_radius(m1::F32_TZ), c->d_nactive};
     {
         Timed t(c->prof, c->stream, K_TILE_MASK);
         hip_check(m1::f32::launch_tile_mask(ma, c->stream), "tile mask kernel");
