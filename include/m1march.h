@@ -245,6 +245,41 @@ int m1_batch_reset_stats(m1_batch *batch);
 /* Host time of the per-step physics scalars since the last reset, ms. */
 int m1_batch_host_ms(m1_batch *batch, double *setup_ms);
 
+/* ------------------------------------------------------------------
+ * Tile-skip network (no reference counterpart): 11 inputs per workgroup
+ * tile, three hidden layers of 64 ReLU units, one logit (probability that
+ * the tile stays empty over the next steps).  Weights are applied in
+ * bfloat16 (rounded to nearest even from the float32 master copy), biases
+ * and accumulations in float32.  Parameters are a flat float32 array of
+ * m1_mlp_param_count() values:
+ *   W1[11][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
+ * (W[i][j]: input i to unit j).  Training (Adam on the mean binary
+ * cross-entropy, shuffled minibatches) runs on the device.
+ * ------------------------------------------------------------------ */
+typedef struct m1_mlp m1_mlp;
+
+#define M1_MLP_INPUTS 11
+
+int32_t m1_mlp_param_count(void);
+int m1_mlp_create(m1_mlp **out, int device);
+void m1_mlp_destroy(m1_mlp *model);
+const char *m1_mlp_last_error(const m1_mlp *model);
+/* Set / read the float32 master parameters (setting resets Adam's moments). */
+int m1_mlp_set_params(m1_mlp *model, const float *params);
+int m1_mlp_get_params(m1_mlp *model, float *params);
+/* Probabilities sigmoid(logit) of n samples x (n, 11). */
+int m1_mlp_forward(m1_mlp *model, const float *x, int32_t n, float *prob);
+/* Mean binary cross-entropy of labels y (0 or 1) and its gradient with
+ * respect to the parameters (weights entering through their bf16 values). */
+int m1_mlp_loss_grad(m1_mlp *model, const float *x, const float *y, int32_t n,
+                     double *loss, float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `epochs` passes of shuffled
+ * minibatches of `batch` samples; *final_loss is the mean loss over the last
+ * pass. */
+int m1_mlp_train(m1_mlp *model, const float *x, const float *y, int32_t n,
+                 int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                 double *final_loss);
+
 
 #ifdef __cplusplus
 }

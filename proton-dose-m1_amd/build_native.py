@@ -53,6 +53,7 @@ def _units():
         (CSRC / "m1_batch_kernels.hip", BUILD / "m1_batch_kernels.o",
          ["-ffp-contract=off", *waves]),
         (CSRC / "m1_batch_march.hip", BUILD / "m1_batch_march.o", ["-ffp-contract=off", *waves]),
+        (CSRC / "m1_mlp.hip", BUILD / "m1_mlp.o", ["-ffp-contract=off"]),
     ]
 
 

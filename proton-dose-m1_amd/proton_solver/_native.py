@@ -32,6 +32,7 @@ STEP_STRANG = 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
 
 
 class m1_table(C.Structure):
@@ -104,6 +105,16 @@ SIGNATURES = [
                                       C.POINTER(C.c_int64)]),
     ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
+    ("m1_mlp_param_count", C.c_int32, []),
+    ("m1_mlp_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("m1_mlp_destroy", None, [C.c_void_p]),
+    ("m1_mlp_last_error", C.c_char_p, [C.c_void_p]),
+    ("m1_mlp_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("m1_mlp_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("m1_mlp_forward", C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    ("m1_mlp_loss_grad", C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, _dp, _fp]),
+    ("m1_mlp_train", C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                               C.c_uint64, _dp]),
 ]
 
 _lib = None
@@ -137,6 +148,8 @@ def _ptr(a):
         return None
     if a.dtype == np.int32:
         return a.ctypes.data_as(_ip)
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(_fp)
     return a.ctypes.data_as(_dp)
 
 
@@ -442,3 +455,79 @@ class Batch:
             self._check(self.lib.m1_batch_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
             out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
         return out
+
+
+class Mlp:
+    """The tile-skip network on one device (m1_mlp_*)."""
+
+    N_INPUTS = 11
+
+    def __init__(self, device: int = 0, params=None):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.m1_mlp_create(C.byref(h), int(device))
+        if rc != M1_OK:
+            raise MarchError(self.lib.m1_mlp_last_error(None).decode())
+        self.h = h
+        self.n_params = int(self.lib.m1_mlp_param_count())
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.m1_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc == M1_OK:
+            return
+        msg = self.lib.m1_mlp_last_error(self.h).decode()
+        if rc == M1_ERR_VALUE:
+            raise ValueError(msg)
+        raise MarchError(msg)
+
+    def _x(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != self.N_INPUTS:
+            raise ValueError(f"x must have shape (n, {self.N_INPUTS})")
+        return x
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (self.n_params,):
+            raise ValueError(f"expected {self.n_params} parameters")
+        self._check(self.lib.m1_mlp_set_params(self.h, _ptr(p)))
+
+    def get_params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        self._check(self.lib.m1_mlp_get_params(self.h, _ptr(p)))
+        return p
+
+    def forward(self, x):
+        x = self._x(x)
+        out = np.empty(len(x), dtype=np.float32)
+        self._check(self.lib.m1_mlp_forward(self.h, _ptr(x), len(x), _ptr(out)))
+        return out
+
+    def loss_grad(self, x, y):
+        x = self._x(x)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_double()
+        g = np.empty(self.n_params, dtype=np.float32)
+        self._check(self.lib.m1_mlp_loss_grad(self.h, _ptr(x), _ptr(y), len(x), C.byref(loss),
+                                              _ptr(g)))
+        return loss.value, g
+
+    def train(self, x, y, epochs=5, batch=4096, lr=1e-3, seed=0):
+        x = self._x(x)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.m1_mlp_train(self.h, _ptr(x), _ptr(y), len(x), int(epochs),
+                                          int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
