@@ -134,21 +134,9 @@ def roofline_of(stats, stage_bytes, cells_per_launch):
 
 
 def batch_config(tile_threshold=1e-10):
-    from proton_solver import (
-        Beam, DeviceOptions, EnergyScheme, Geometry, Numerics, Physics, Precision, Scattering,
-        SolverConfig, SpatialScheme, WaveSpeeds,
-    )
+    from proton_solver.batch import batch_workload_config
 
-    # geometry and beam are placeholders: the batch takes the phantom and beamlets
-    return SolverConfig(
-        geometry=Geometry(size=(0.2, 0.2, 0.3), resolution=(200, 200, 300)),
-        beam=Beam(energy=150.0),
-        physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
-        numerics=Numerics(spatial_scheme=SpatialScheme.WENO_Z,
-                          energy_scheme=EnergyScheme.IF_SSPRK3, wave_speeds=WaveSpeeds.EXACT,
-                          max_energy_step=0.25, cfl=0.9, energy_min=1.0),
-        device=DeviceOptions(precision=Precision.FP32_BULK, tile_threshold=tile_threshold),
-    )
+    return batch_workload_config(tile_threshold)
 
 
 def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_beamlet):
@@ -195,6 +183,13 @@ def run_batch(args):
     handle, boxes = prepare_batch(phantom, beamlets, cfg, boxes)
     setup_s = time.perf_counter() - t0
     num = _native.numerics_struct(cfg)
+    from proton_solver import tileskip
+
+    use_net = args.skip_network == "on" or (args.skip_network == "auto" and
+                                            tileskip.WEIGHTS_FILE.exists())
+    net_opts = tileskip.SkipOptions()
+    model = tileskip.load_model(0) if use_net else None
+    tileskip.enable(handle, model, net_opts)
     for _ in range(args.warmup):
         handle.march(num, want_dose=False)
     handle.set_profiling(True)
@@ -209,16 +204,48 @@ def run_batch(args):
     handle.set_profiling(False)
     K = max(args.steps, 1)
     B = len(beamlets)
+
+    # The network's effect: the same batch marched with the dilation rule alone
+    # (outside the timed region).
+    net = None
+    beamlet_dose = handle.beamlet_dose
+    if use_net:
+        judged, rejections, back = handle.skip_stats()
+        net_doses = [handle.beamlet_dose(b) for b in range(B)]
+        beamlet_dose = net_doses.__getitem__
+        net = {"weights": tileskip.WEIGHTS_FILE.name, "decision": net_opts.decision,
+               "horizon": net_opts.horizon, "judge_every": net_opts.judge_every,
+               "judge_tol": net_opts.judge_tol, "judged_steps": judged,
+               "rejected_beamlets": rejections}
+        if not args.no_network_reference:
+            tileskip.enable(handle, None)
+            handle.reset_stats()
+            t1 = time.perf_counter()
+            _, steps0 = handle.march(num, want_dose=False)
+            sec0 = time.perf_counter() - t1
+            ev0, _ = handle.tile_stats()
+            worst = 0.0
+            for b in range(B):
+                ref = handle.beamlet_dose(b)
+                worst = max(worst, float(np.abs(net_doses[b] - ref).max() / ref.max()))
+            net.update({
+                "dilation_only_seconds": round(sec0, 3),
+                "dilation_only_beamlets_per_s": B / sec0,
+                "tiles_vs_dilation_only": ev / K / ev0,
+                "worst_beamlet_max_dose_change_over_max": worst,
+            })
+        print(json.dumps({"tile_skip_network": net}), file=sys.stderr, flush=True)
     box_cells = np.array([b[2] * b[3] * b[4] for b in boxes], dtype=np.float64)
-    tile_cap = np.array([-(-b[2] // 16) * -(-b[3] // 8) * -(-b[4] // 32) * 4096 for b in boxes],
-                        dtype=np.float64)
+    tx, ty, tz = _native.tile_shape()
+    tile_cap = np.array([-(-b[2] // tx) * -(-b[3] // ty) * -(-b[4] // tz) * (tx * ty * tz)
+                         for b in boxes], dtype=np.float64)
     fill = box_cells.sum() / tile_cap.sum()
     cell_steps = float((box_cells * steps).sum()) * K
     n_phantom = int(np.prod(phantom.shape))
     eval_frac = ev / full if full else 1.0
 
     def cells_per_launch(launches):
-        return ev * 4096.0 * fill / launches
+        return ev * float(tx * ty * tz) * fill / launches
 
     roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch)
 
@@ -240,7 +267,7 @@ def run_batch(args):
             ref = ctx.get_dose()
             x0, y0, nx, ny, nz = boxes[b]
             got = np.zeros(phantom.shape)
-            got[x0:x0 + nx, y0:y0 + ny, :nz] = handle.beamlet_dose(b)
+            got[x0:x0 + nx, y0:y0 + ny, :nz] = beamlet_dose(b)
             # gamma on the box plus a 4-cell margin (the FP64 dose beyond it is < 1e-3 of max)
             sl = (slice(max(x0 - 4, 0), x0 + nx + 4), slice(max(y0 - 4, 0), y0 + ny + 4),
                   slice(0, min(nz + 4, phantom.shape[2])))
@@ -292,7 +319,8 @@ def run_batch(args):
         "config": {
             "workload": "256 pencil beamlets in one batch, 200x200x300 @1 mm water phantom with "
                         "12 bone/lung ellipsoids, 64x64xdepth boxes, Wentzel+nuclear, IF-SSPRK3, "
-                        "FP32 bulk with tile skipping",
+                        "FP32 bulk with tile skipping" +
+                        (" and the tile-skip network" if use_net else ""),
             "n_beamlets": B,
             "box_cells_total": int(box_cells.sum()),
             "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
@@ -303,6 +331,7 @@ def run_batch(args):
         "cell_steps_per_s_evaluated": cell_steps * eval_frac / elapsed,
         "full_domain_equivalent_cell_steps_per_s": float(n_phantom * steps.sum()) * K / elapsed,
         "tiles_evaluated_fraction": eval_frac,
+        "tile_skip_network": net,
         "host_step_setup_ms_per_batch": host_ms / K,
         "setup_s": setup_s,
         "kernel_ms_per_step": {k: round(v[1] / K, 3) for k, v in stats.items() if v[0]},
@@ -328,6 +357,10 @@ def main():
     ap.add_argument("--beamlets", type=int, default=256)
     ap.add_argument("--fp64-sample", type=int, default=2)
     ap.add_argument("--tile-threshold", type=float, default=1e-10)
+    ap.add_argument("--skip-network", default="auto", choices=["auto", "on", "off"],
+                    help="tile-skip network in the batched march (auto: when its weights exist)")
+    ap.add_argument("--no-network-reference", action="store_true",
+                    help="skip the untimed march without the network")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

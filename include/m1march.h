@@ -187,6 +187,8 @@ int m1_reset_stats(m1_ctx *ctx);
 /* Tiles evaluated by the stage kernels since the last m1_reset_stats, and the
  * number a full-domain march would have evaluated. */
 int m1_tile_stats(m1_ctx *ctx, int64_t *evaluated, int64_t *full);
+/* Cells per FP32-bulk workgroup tile along x, y and z in this build. */
+void m1_tile_shape(int32_t *tx, int32_t *ty, int32_t *tz);
 
 /* ------------------------------------------------------------------
  * Batched beamlets.  No reference counterpart: the reference solves one
@@ -279,6 +281,53 @@ int m1_mlp_loss_grad(m1_mlp *model, const float *x, const float *y, int32_t n,
 int m1_mlp_train(m1_mlp *model, const float *x, const float *y, int32_t n,
                  int32_t epochs, int32_t batch, float lr, uint64_t seed,
                  double *final_loss);
+
+/* ------------------------------------------------------------------
+ * Tile-skip network inside the batched march (no reference counterpart).
+ *
+ * Features of a tile (computed on the device from the last step):
+ *   0  log10(max phi) / 10          1  log10(mean phi) / 10
+ *   2  sum |J| / sum phi            3  mean density / 1000 (g/cm^3)
+ *   4  beamlet energy / 250 MeV     5-10  log10(max phi) / 10 of the six
+ *                                         face neighbours (-x +x -y +y -z +z)
+ * (phi below 1e-30 counts as 1e-30; tiles not evaluated count as empty).
+ *
+ * Label: 1 when a tile that only the dilation adds to the set (its own phi
+ * is below the tile threshold, it is not an inflow tile) keeps phi at or
+ * below the threshold over the next `horizon` steps.
+ *
+ * With a model set, the network is evaluated after every step on the tiles
+ * that step evaluated; a tile whose probability exceeds `decision` is left
+ * out for the next `horizon` steps (1..255) whenever only the dilation would
+ * add it.  Tiles above the threshold and inflow tiles are never left out.
+ * Every `judge_every` (0: never)
+ * steps of a beamlet, and on its last step, the judge takes that step twice
+ * from the same state: on every tile of the beamlet's box and on the tile set
+ * in use.  When the two step doses differ anywhere by more than judge_tol
+ * times the step's largest dose increment, the beamlet goes back to the state
+ * and dose of its last accepted judge (or to its start) and marches on from
+ * there without the network; otherwise that state becomes its checkpoint.
+ * The model must stay alive, unchanged, while the batch uses it; marches
+ * need tile_threshold > 0.
+ * ------------------------------------------------------------------ */
+/* model NULL switches the network off. */
+int m1_batch_set_skip_model(m1_batch *batch, const m1_mlp *model, float decision,
+                            int32_t horizon, int32_t judge_every, float judge_tol);
+/* Record (features, label) samples of the dilation-only tiles during the
+ * following marches, every `horizon` steps, up to `capacity` samples.
+ * full_domain: evaluate every tile of the active boxes (labels from the
+ * unskipped march).  enable 0 stops recording and frees the samples. */
+int m1_batch_record(m1_batch *batch, int32_t enable, int32_t full_domain,
+                    int32_t horizon, int64_t capacity);
+/* *count = samples held; copies min(count, capacity) of them into x
+ * (count, 11) and y. */
+int m1_batch_samples(m1_batch *batch, float *x, float *y, int64_t capacity,
+                     int64_t *count);
+/* Judge checks made and beamlets rejected since the model was set;
+ * rollback_to (nullable) receives, per beamlet, the step count it was sent
+ * back to in the last march (-1: never rejected). */
+int m1_batch_skip_stats(m1_batch *batch, int64_t *judged, int64_t *rejections,
+                        int32_t *rollback_to);
 
 
 #ifdef __cplusplus

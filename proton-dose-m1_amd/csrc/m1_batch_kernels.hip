@@ -9,6 +9,7 @@
 // XCDs (one L2 each) takes one contiguous slice of the list, keeping the
 // halo re-reads of neighbouring tiles in the same L2.
 #include "m1_f32_common.h"
+#include "m1_mlp.h"
 
 namespace m1 {
 namespace batch {
@@ -47,18 +48,25 @@ __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArg
     const TileCoord tc = tile_coord(d, b, t);
     const long off4 = 4 * d.cell_off;
     const long m2 = long(d.nx) * d.ny;
+    float* dose_hi = a.dose_hi;
+    float* dose_lo = a.dose_lo;
+    if (dose_hi && a.judge && a.judge[b]) {
+        dose_hi = a.judge_hi;
+        dose_lo = a.judge_lo;
+    }
     TileView v{{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz},
                a.src + off4,
                a.U + off4,
                a.V1 ? a.V1 + off4 : nullptr,
                a.out ? a.out + off4 : nullptr,
-               a.dose_hi ? a.dose_hi + d.cell_off : nullptr,
-               a.dose_lo ? a.dose_lo + d.cell_off : nullptr,
+               dose_hi ? dose_hi + d.cell_off : nullptr,
+               dose_lo ? dose_lo + d.cell_off : nullptr,
                nullptr,
                a.ids + d.cell_off,
                a.bnd + 12 * d.col_off + 4 * m2 * a.bnd_stage,
                a.coef + b,
                a.tile_max ? a.tile_max + t : nullptr,
+               a.tile_sums ? a.tile_sums + 2 * long(t) : nullptr,
                a.wave_mode,
                tc.tx,
                tc.ty,
@@ -138,7 +146,11 @@ __global__ void boundary_kernel(BatchBoundaryArgs a)
 
 // Next-step tile set of every active box: the single-box rule (a tile is
 // evaluated when a tile within the dilation radius holds phi above the
-// threshold, or it is a z = 0 tile while the inflow is on) per box.
+// threshold, or it is a z = 0 tile while the inflow is on) per box.  The
+// tile-skip network may leave out, for the steps its countdown runs, tiles
+// that only the dilation adds (never a tile above the threshold or one fed
+// by the inflow).  Also records, when asked, the tiles the dilation adds
+// (ring) and the running max of tile_max.
 __global__ void mask_kernel(BatchMaskArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,17 +158,19 @@ __global__ void mask_kernel(BatchMaskArgs a)
         return;
     const int b = a.tile_box[t];
     const BoxStep st = a.step[b];
-    if (!st.active) {
+    if (!st.active || (a.only && !a.only[b])) {
         a.next[t] = 0;
-        return;
-    }
-    if (a.all_on) {
-        a.next[t] = 1;
+        if (a.ring)
+            a.ring[t] = 0;
         return;
     }
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
-    bool on = st.inflow_on && tc.tz == 0;
+    const float self_max = a.computed[t] ? a.tile_max[t] : 0.f;
+    if (a.wmax)
+        a.wmax[t] = fmaxf(a.wmax[t], self_max);
+    const bool forced = st.inflow_on && tc.tz == 0;
+    bool on = forced;
     for (int dx = -a.rx; dx <= a.rx && !on; ++dx) {
         const int xx = tc.tx + dx;
         if (xx < 0 || xx >= d.ntx)
@@ -174,7 +188,169 @@ __global__ void mask_kernel(BatchMaskArgs a)
             }
         }
     }
-    a.next[t] = on ? 1 : 0;
+    const bool self_on = self_max > a.threshold;
+    if (a.ring)
+        a.ring[t] = (on && !forced && !self_on) ? 1 : 0;
+    if (a.pred && a.pred[t] > 0) {
+        if (on && !forced && !self_on && st.model_on)
+            on = false;
+        a.pred[t] -= 1;
+    }
+    a.next[t] = (a.all_on || st.full_set || on) ? 1 : 0;
+}
+
+// Features of every tile of the active boxes and, with a model, the
+// network's prediction for the tiles evaluated in the last step.  One wave
+// per 64 tiles; the network's activations live in LDS (m1_mlp.h).
+__global__ void __launch_bounds__(mlp::WG) feature_kernel(BatchFeatureArgs a)
+{
+    __shared__ uint16_t wl[mlp::NPARAM];
+    __shared__ float A[mlp::NH][mlp::WG], B[mlp::NH][mlp::WG];
+    if (a.w16) {
+        for (int k = threadIdx.x; k < mlp::NPARAM; k += blockDim.x)
+            wl[k] = a.w16[k];
+        __syncthreads();
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_tiles)
+        return;  // no barrier follows
+    const int b = a.tile_box[t];
+    const BoxStep st = a.step[b];
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    auto lg = [](float v) { return log10f(fmaxf(v, 1e-30f)) * 0.1f; };
+    auto tmax = [&](int u) { return a.computed[u] ? a.tile_max[u] : 0.f; };
+    const float mx = tmax(t);
+    const float ps = a.computed[t] ? a.tile_sums[2 * long(t)] : 0.f;
+    const float js = a.computed[t] ? a.tile_sums[2 * long(t) + 1] : 0.f;
+    float x[mlp::NIN];
+    x[0] = lg(mx);
+    x[1] = lg(ps * (1.f / float(TX * TY * TZ)));
+    x[2] = ps > 0.f ? fminf(js / ps, 1.f) : 0.f;
+    x[3] = a.tile_rho[t] * 1e-3f;
+    x[4] = st.energy * (1.f / 250.f);
+    const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int xx = tc.tx + nb[k][0], yy = tc.ty + nb[k][1], zz = tc.tz + nb[k][2];
+        const bool in = xx >= 0 && xx < d.ntx && yy >= 0 && yy < d.nty && zz >= 0 && zz < d.ntz;
+        x[5 + k] = in ? lg(tmax(d.tile_off + (xx * d.nty + yy) * d.ntz + zz)) : lg(0.f);
+    }
+    if (a.features && st.active)
+#pragma unroll
+        for (int k = 0; k < mlp::NIN; ++k)
+            a.features[long(t) * mlp::NIN + k] = x[k];
+    const bool forced = st.inflow_on && tc.tz == 0;
+    if (a.w16 && a.computed[t] && st.active && st.model_on) {
+        bool drop = false;
+        if (!forced && !(mx > a.threshold)) {
+            const float z = mlp::forward_one(wl, a.params, x, A, B, threadIdx.x, nullptr,
+                                             nullptr, nullptr);
+            drop = mlp::sigmoid(z) > a.decision;
+        }
+        a.pred[t] = drop ? uint8_t(a.horizon) : uint8_t(0);
+    }
+}
+
+// Recording: one sample per ring tile saved `horizon` steps ago.
+__global__ void label_kernel(BatchLabelArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_tiles || !a.cand[t])
+        return;
+    const int k = atomicAdd(a.count, 1);
+    if (k >= a.cap)
+        return;
+    for (int i = 0; i < mlp::NIN; ++i)
+        a.out_x[long(k) * mlp::NIN + i] = a.features[long(t) * mlp::NIN + i];
+    a.out_y[k] = a.wmax[t] <= a.threshold ? 1.f : 0.f;
+}
+
+// Judge: largest |full - skip| and |full| dose increment of each box.
+__global__ void compare_kernel(BatchCompareArgs a)
+{
+    __shared__ float red[2][256 / 64];
+    const int b = blockIdx.y;
+    if (!a.judge[b])
+        return;
+    const BoxDesc d = a.box[b];
+    const long n = long(d.nx) * d.ny * d.nz;
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    float df = 0.f, fl = 0.f;
+    if (i < n) {
+        const long c = d.cell_off + i;
+        const float f = a.full_hi[c] + a.full_lo[c];
+        const float k = a.skip_hi[c] + a.skip_lo[c];
+        df = fabsf(f - k);
+        fl = fabsf(f);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        df = fmaxf(df, __shfl_xor(df, off, 64));
+        fl = fmaxf(fl, __shfl_xor(fl, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = df;
+        red[1][threadIdx.x >> 6] = fl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < int(blockDim.x / 64); ++w) {
+            df = fmaxf(df, red[0][w]);
+            fl = fmaxf(fl, red[1][w]);
+        }
+        df = fmaxf(df, red[0][0]);
+        fl = fmaxf(fl, red[1][0]);
+        atomicMax(a.max_diff + b, __float_as_uint(df));
+        atomicMax(a.max_full + b, __float_as_uint(fl));
+    }
+}
+
+__global__ void slices_kernel(BatchSliceArgs a)
+{
+    const int b = blockIdx.y;
+    const int sel = a.sel[b];
+    if (!sel)
+        return;
+    const BoxDesc d = a.box[b];
+    const long n = long(d.nx) * d.ny * d.nz;
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const long c = d.cell_off + i;
+    switch (a.op) {
+    case SLICE_SAVE: {
+        const int k = sel - 1;
+        for (int m = 0; m < 4; ++m)
+            a.ck_U[k][4 * d.cell_off + m * n + i] = a.U[4 * d.cell_off + m * n + i];
+        a.ck_hi[k][c] = a.dose_hi[c];
+        a.ck_lo[k][c] = a.dose_lo[c];
+        break;
+    }
+    case SLICE_RESTORE: {
+        if (sel == 3) {
+            for (int m = 0; m < 4; ++m)
+                a.U[4 * d.cell_off + m * n + i] = 0.f;
+            a.dose_hi[c] = 0.f;
+            a.dose_lo[c] = 0.f;
+        } else {
+            const int k = sel - 1;
+            for (int m = 0; m < 4; ++m)
+                a.U[4 * d.cell_off + m * n + i] = a.ck_U[k][4 * d.cell_off + m * n + i];
+            a.dose_hi[c] = a.ck_hi[k][c];
+            a.dose_lo[c] = a.ck_lo[k][c];
+        }
+        break;
+    }
+    case SLICE_RESET:
+        a.full_hi[c] = a.full_lo[c] = a.skip_hi[c] = a.skip_lo[c] = 0.f;
+        break;
+    default: {
+        float hi = a.dose_hi[c], lo = a.dose_lo[c];
+        two_sum_add(hi, lo, a.skip_hi[c]);
+        a.dose_hi[c] = hi;
+        a.dose_lo[c] = lo + a.skip_lo[c];
+    }
+    }
 }
 
 // One workgroup: ascending list of the set flags (each thread scans one
@@ -336,6 +512,41 @@ hipError_t launch_scatter_dose(const float* hi, const float* lo, int nx, int ny,
     const long n = long(nx) * ny * nz;
     hipLaunchKernelGGL(scatter_dose_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, hi,
                        lo, nx, ny, nz, x0, y0, py, pz, phantom);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const BatchFeatureArgs& a, hipStream_t s)
+{
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(feature_kernel, dim3(unsigned((a.n_tiles + mlp::WG - 1) / mlp::WG)),
+                       dim3(mlp::WG), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_labels(const BatchLabelArgs& a, hipStream_t s)
+{
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(label_kernel, dim3(unsigned((a.n_tiles + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_compare(const BatchCompareArgs& a, int max_cells, hipStream_t s)
+{
+    if (a.n_boxes <= 0 || max_cells <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(compare_kernel, dim3(unsigned((max_cells + 255) / 256), unsigned(a.n_boxes)),
+                       dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_slices(const BatchSliceArgs& a, int max_cells, hipStream_t s)
+{
+    if (a.n_boxes <= 0 || max_cells <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(slices_kernel, dim3(unsigned((max_cells + 255) / 256), unsigned(a.n_boxes)),
+                       dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

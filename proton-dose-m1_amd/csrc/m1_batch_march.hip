@@ -11,6 +11,17 @@
 //      (box, tile) list;
 //   4. per-box tile sets of the next step (mask, compaction, zeroing).
 // The host synchronises once per iteration (step 1's read-back).
+//
+// Tile-skip network (optional, m1_batch_set_skip_model): every `horizon`
+// iterations the network sees the features of every tile and marks tiles it
+// expects to stay empty; for the next `horizon` steps the mask drops those
+// of them that only the dilation would add.  Every `judge_every` iterations
+// the step is also taken on the full box from the same state, and a beamlet
+// whose step dose differs by more than judge_tol of that step's largest dose
+// increment falls back to the plain mask for the rest of its march.
+// Recording (m1_batch_record) collects the network's training samples from
+// plain runs: the features of the tiles the dilation adds, labelled 1 when
+// the tile stays at or below the threshold for the next `horizon` steps.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -19,6 +30,7 @@
 #include <vector>
 
 #include "m1_host.h"
+#include "m1_mlp.h"
 
 using namespace m1;
 
@@ -66,16 +78,86 @@ struct m1_batch {
     unsigned long long* h_red = nullptr;
     int* h_count = nullptr;
 
+    float* d_tile_rho = nullptr;     // mean density per tile
+    float* d_tile_sums = nullptr;    // sum phi, sum |J| per tile (stage 2)
+    uint8_t* d_pred = nullptr;       // network: tiles expected to stay empty
+    uint8_t* d_ring = nullptr;       // tiles only the dilation adds
+    uint8_t* d_cand = nullptr;       // ring tiles saved for labelling
+    float* d_wmax = nullptr;         // running max of tile_max
+    float* d_feat = nullptr;         // features saved for labelling
+    // skip network and its judge
+    const m1_mlp* model = nullptr;
+    float decision = 0.5f, judge_tol = 1e-3f;
+    int horizon = 8, judge_every = 64;
+    std::vector<uint8_t> rejected;
+    std::vector<int32_t> rollback_to;  // step a rejected beamlet restarted from (-1: none)
+    // two checkpoints per box (state and dose before a judged step)
+    float *ck_U[2] = {nullptr, nullptr}, *ck_hi[2] = {nullptr, nullptr},
+          *ck_lo[2] = {nullptr, nullptr};
+    float *jf_hi = nullptr, *jf_lo = nullptr, *js_hi = nullptr, *js_lo = nullptr;
+    uint8_t* d_flags_full = nullptr;
+    int* d_list_full = nullptr;
+    int* d_count_full = nullptr;
+    unsigned *d_jdiff = nullptr, *d_jfull = nullptr, *h_jdiff = nullptr, *h_jfull = nullptr;
+    uint8_t *d_sel = nullptr, *h_sel = nullptr;  // judge flags | save selectors | roll-back
+    int64_t judged = 0, rejections = 0;
+    // recording
+    bool recording = false, record_full = false;
+    int record_horizon = 8;
+    int record_cap = 0;
+    float *d_rx = nullptr, *d_ry = nullptr;
+    int* d_rcount = nullptr;
+
     Profiler prof;
     double host_ms = 0.0;
     bool marched = false;
+
+    void free_model_buffers();
+    void free_record_buffers();
 
     void free_problem();
     ~m1_batch();
 };
 
+void m1_batch::free_model_buffers()
+{
+    void* dev[] = {ck_U[0], ck_U[1], ck_hi[0], ck_hi[1], ck_lo[0], ck_lo[1], jf_hi, jf_lo,
+                   js_hi, js_lo, d_flags_full, d_list_full, d_count_full, d_jdiff, d_jfull,
+                   d_sel};
+    for (void* p : dev)
+        if (p)
+            (void)hipFree(p);
+    for (void* p : {(void*)h_jdiff, (void*)h_jfull, (void*)h_sel})
+        if (p)
+            (void)hipHostFree(p);
+    for (int k = 0; k < 2; ++k)
+        ck_U[k] = ck_hi[k] = ck_lo[k] = nullptr;
+    jf_hi = jf_lo = js_hi = js_lo = nullptr;
+    d_flags_full = d_sel = h_sel = nullptr;
+    d_list_full = d_count_full = nullptr;
+    d_jdiff = d_jfull = h_jdiff = h_jfull = nullptr;
+}
+
+void m1_batch::free_record_buffers()
+{
+    for (void* p : {(void*)d_rx, (void*)d_ry, (void*)d_rcount})
+        if (p)
+            (void)hipFree(p);
+    d_rx = d_ry = nullptr;
+    d_rcount = nullptr;
+    record_cap = 0;
+}
+
 void m1_batch::free_problem()
 {
+    free_model_buffers();
+    free_record_buffers();
+    for (void* p : {(void*)d_tile_rho, (void*)d_tile_sums, (void*)d_pred, (void*)d_ring,
+                    (void*)d_cand, (void*)d_wmax, (void*)d_feat})
+        if (p)
+            (void)hipFree(p);
+    d_tile_rho = d_tile_sums = d_wmax = d_feat = nullptr;
+    d_pred = d_ring = d_cand = nullptr;
     void* dev[] = {d_box,  d_step,   d_tile_box, d_tile_mats, U,         V1,       V2,
                    dose_hi, dose_lo, bnd,        d_tile_max,  ids,       spatial,  d_flags[0],
                    d_flags[1], d_list, d_count,  d_coef,      d_cfl,     d_coef32, d_red,
@@ -275,6 +357,8 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         std::vector<uint8_t> box_ids(static_cast<size_t>(cells));
         std::vector<int> tile_box(static_cast<size_t>(tiles));
         std::vector<uint8_t> tile_mats(static_cast<size_t>(tiles), 0);
+        std::vector<float> tile_rho(static_cast<size_t>(tiles), 0.f);
+        std::vector<int> tile_cells(static_cast<size_t>(tiles), 0);
         for (int b = 0; b < n_beamlets; ++b) {
             const m1_beamlet& q = beamlets[b];
             const BoxDesc& d = boxes[size_t(b)];
@@ -287,10 +371,16 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
                         const int t = d.tile_off +
                                       ((x / F32_TX) * d.nty + y / F32_TY) * d.ntz + z / F32_TZ;
                         tile_mats[size_t(t)] |= uint8_t(1u << src[z]);
+                        tile_rho[size_t(t)] += float(mats[size_t(src[z])].density);
+                        tile_cells[size_t(t)] += 1;
                     }
                 }
             for (int t = 0; t < d.ntx * d.nty * d.ntz; ++t)
                 tile_box[size_t(d.tile_off + t)] = b;
+        }
+        for (size_t t = 0; t < tile_rho.size(); ++t) {
+            if (tile_cells[t] > 0)
+                tile_rho[t] /= float(tile_cells[t]);
         }
 
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -333,6 +423,15 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         dev_alloc(&c->d_coef32, B, "hipMalloc batch coefficients (f32)");
         dev_alloc(&c->d_red, B, "hipMalloc batch CFL maxima");
         dev_alloc(&c->d_box_dose, size_t(max_cells), "hipMalloc beamlet dose");
+        dev_alloc(&c->d_tile_rho, T, "hipMalloc tile density");
+        dev_alloc(&c->d_tile_sums, 2 * T, "hipMalloc tile sums");
+        dev_alloc(&c->d_pred, T, "hipMalloc tile predictions");
+        dev_alloc(&c->d_ring, T, "hipMalloc tile ring");
+        dev_alloc(&c->d_cand, T, "hipMalloc tile candidates");
+        dev_alloc(&c->d_wmax, T, "hipMalloc tile window max");
+        dev_alloc(&c->d_feat, T * mlp::NIN, "hipMalloc tile features");
+        c->rejected.assign(B, 0);
+        c->rollback_to.assign(B, -1);
         host_alloc(&c->h_coef, B, "hipHostMalloc");
         host_alloc(&c->h_cfl, B, "hipHostMalloc");
         host_alloc(&c->h_coef32, B, "hipHostMalloc");
@@ -345,6 +444,9 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
                             hipMemcpyHostToDevice),
                   "tile upload");
         hip_check(hipMemcpy(c->d_tile_mats, tile_mats.data(), T, hipMemcpyHostToDevice),
+                  "tile upload");
+        hip_check(hipMemcpy(c->d_tile_rho, tile_rho.data(), T * sizeof(float),
+                            hipMemcpyHostToDevice),
                   "tile upload");
         hip_check(hipMemcpy(c->ids, box_ids.data(), N, hipMemcpyHostToDevice), "ids upload");
         hip_check(hipMemcpy(c->spatial, inflow_spatial, size_t(cols) * sizeof(double),
@@ -387,9 +489,60 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             const BoxDesc& d = c->boxes[size_t(b)];
             tiles_of[size_t(b)] = d.ntx * d.nty * d.ntz;
             const bool on = E[size_t(b)] > n.emin;
-            c->h_step[b] = {on ? 1 : 0, 1};
+            c->rejected[size_t(b)] = 0;
+            c->rollback_to[size_t(b)] = -1;
+            c->h_step[b] = {on ? 1 : 0, 1, c->model ? 1 : 0, float(E[size_t(b)]), 0};
             n_active += on ? 1 : 0;
         }
+        if ((c->model || c->recording) && !(n.tile_threshold > 0.0))
+            throw ValueError{"the tile-skip network and recording need tile_threshold > 0"};
+        const bool use_model = c->model != nullptr;
+        const bool judging = use_model && c->judge_every > 0;
+        if (judging && !c->d_sel) {
+            for (int k = 0; k < 2; ++k) {
+                dev_alloc(&c->ck_U[k], 4 * N, "hipMalloc checkpoint state");
+                dev_alloc(&c->ck_hi[k], N, "hipMalloc checkpoint dose");
+                dev_alloc(&c->ck_lo[k], N, "hipMalloc checkpoint dose");
+            }
+            dev_alloc(&c->jf_hi, N, "hipMalloc judge dose");
+            dev_alloc(&c->jf_lo, N, "hipMalloc judge dose");
+            dev_alloc(&c->js_hi, N, "hipMalloc judge dose");
+            dev_alloc(&c->js_lo, N, "hipMalloc judge dose");
+            dev_alloc(&c->d_flags_full, T, "hipMalloc judge tile set");
+            dev_alloc(&c->d_list_full, T, "hipMalloc judge tile list");
+            hip_check(hipMemset(c->d_list_full, 0, T * sizeof(int)), "judge tile list");
+            dev_alloc(&c->d_count_full, 1, "hipMalloc judge tile count");
+            dev_alloc(&c->d_jdiff, nb, "hipMalloc judge maxima");
+            dev_alloc(&c->d_jfull, nb, "hipMalloc judge maxima");
+            host_alloc(&c->h_jdiff, nb, "hipHostMalloc");
+            host_alloc(&c->h_jfull, nb, "hipHostMalloc");
+            dev_alloc(&c->d_sel, 3 * nb, "hipMalloc judge flags");
+            host_alloc(&c->h_sel, 3 * nb, "hipHostMalloc");
+        }
+        // Judge bookkeeping: acc[b] is the checkpoint (0 or 1) that passed the
+        // last judge, -1 for the initial state; since[b] counts the steps
+        // taken after it.
+        std::vector<int> acc(nb, -1), since(nb, 0), pend(nb, 0);
+        std::vector<double> ckE[2] = {std::vector<double>(nb), std::vector<double>(nb)};
+        std::vector<int32_t> ckS[2] = {std::vector<int32_t>(nb), std::vector<int32_t>(nb)};
+        const std::vector<double> E0 = E;
+        uint8_t* h_judge = c->h_sel;
+        uint8_t* h_save = c->h_sel + nb;
+        uint8_t* h_back = c->h_sel + 2 * nb;
+        auto slices = [&](int op, const uint8_t* sel) {
+            BatchSliceArgs sa{c->d_box,  sel,      B,        op,       c->U,
+                              c->dose_hi, c->dose_lo, {c->ck_U[0], c->ck_U[1]},
+                              {c->ck_hi[0], c->ck_hi[1]}, {c->ck_lo[0], c->ck_lo[1]},
+                              c->jf_hi,  c->jf_lo, c->js_hi, c->js_lo};
+            hip_check(batch::launch_slices(sa, int(c->max_box_cells), s), "judge slice kernel");
+        };
+        if (c->recording)
+            hip_check(hipMemsetAsync(c->d_rcount, 0, sizeof(int), s), "sample count reset");
+        hip_check(hipMemsetAsync(c->d_tile_max, 0, T * sizeof(float), s), "tile max reset");
+        hip_check(hipMemsetAsync(c->d_tile_sums, 0, 2 * T * sizeof(float), s), "tile sum reset");
+        hip_check(hipMemsetAsync(c->d_pred, 0, T, s), "prediction reset");
+        hip_check(hipMemsetAsync(c->d_wmax, 0, T * sizeof(float), s), "window max reset");
+        hip_check(hipMemsetAsync(c->d_cand, 0, T, s), "candidate reset");
         for (float* p : {c->U, c->V1, c->V2})
             hip_check(hipMemsetAsync(p, 0, 4 * N * sizeof(float), s), "state reset");
         hip_check(hipMemsetAsync(c->dose_hi, 0, N * sizeof(float), s), "dose reset");
@@ -402,15 +555,17 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                   "box flag upload");
         {
             // Boxes with nothing to march leave the list at once.
-            BatchMaskArgs ma{c->d_box, c->d_step, c->d_tile_box, int(T), c->d_tile_max,
-                             c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1};
+            BatchMaskArgs ma{c->d_box,      c->d_step, c->d_tile_box, int(T), c->d_tile_max,
+                             c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1,
+                             nullptr,       nullptr,   nullptr,   nullptr};
             hip_check(batch::launch_mask(ma, s), "tile mask kernel");
             cur = 1 - cur;
             hip_check(batch::launch_compact(c->d_flags[cur], int(T), c->d_list, c->d_count, s),
                       "tile compaction kernel");
         }
         const size_t nm = c->mats.size();
-        const bool skipping = n.tile_threshold > 0.0;
+        const bool skipping = n.tile_threshold > 0.0 && !(c->recording && c->record_full);
+        int64_t iter = 0;
 
         while (n_active > 0) {
             // 1. CFL maxima of every active box.
@@ -461,6 +616,26 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 c->prof.tiles_full += tiles_of[size_t(b)];
             }
             c->host_ms += ms_since(t0);
+            // Beamlets judged this step: every judge_every steps after their
+            // checkpoint, and on their last step.
+            int n_judge = 0, n_full = 0;
+            if (judging)
+                for (int b = 0; b < B; ++b) {
+                    const size_t k = size_t(b);
+                    h_judge[b] = h_save[b] = 0;
+                    if (!c->h_step[b].active || !c->h_step[b].model_on)
+                        continue;
+                    const bool last = !(E[k] - dE[k] > n.emin);
+                    if (since[k] + 1 < c->judge_every && !last)
+                        continue;
+                    pend[k] = acc[k] == 0 ? 1 : 0;
+                    ckE[pend[k]][k] = E[k];
+                    ckS[pend[k]][k] = steps[k];
+                    h_judge[b] = 1;
+                    h_save[b] = uint8_t(1 + pend[k]);
+                    n_judge += 1;
+                    n_full += tiles_of[k];
+                }
             hip_check(hipMemcpyAsync(c->d_coef, c->h_coef, size_t(B) * sizeof(StepCoef),
                                      hipMemcpyHostToDevice, s),
                       "coefficient upload");
@@ -478,8 +653,6 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             BatchStageArgs a{};
             a.box = c->d_box;
             a.tile_box = c->d_tile_box;
-            a.list = c->d_list;
-            a.n_list = n_list;
             a.dx = c->dx;
             a.dy = c->dy;
             a.dz = c->dz;
@@ -488,56 +661,161 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             a.ids = c->ids;
             a.bnd = c->bnd;
             a.coef = c->d_coef32;
-            a.tile_max = skipping ? c->d_tile_max : nullptr;
+            a.tile_max = skipping || c->recording ? c->d_tile_max : nullptr;
+            a.tile_sums = use_model || c->recording ? c->d_tile_sums : nullptr;
             a.wave_mode = n.wave;
             a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
-            {
-                a.src = c->U;
-                a.out = c->V1;
-                a.bnd_stage = 0;
-                Timed t(c->prof, s, K_STAGE0);
-                hip_check(batch::launch_stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
-            }
-            {
-                a.src = c->V1;
-                a.out = c->V2;
-                a.bnd_stage = 1;
-                Timed t(c->prof, s, K_STAGE1);
-                hip_check(batch::launch_stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
-            }
-            {
-                a.src = c->V2;
-                a.out = nullptr;
-                a.dose_hi = c->dose_hi;
-                a.dose_lo = c->dose_lo;
-                a.bnd_stage = 2;
-                Timed t(c->prof, s, K_STAGE2);
-                hip_check(batch::launch_stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
+            auto stages = [&](const int* list, int count, float* dhi, float* dlo) {
+                a.list = list;
+                a.n_list = count;
+                {
+                    a.src = c->U;
+                    a.out = c->V1;
+                    a.dose_hi = a.dose_lo = nullptr;
+                    a.bnd_stage = 0;
+                    Timed t(c->prof, s, K_STAGE0);
+                    hip_check(batch::launch_stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
+                }
+                {
+                    a.src = c->V1;
+                    a.out = c->V2;
+                    a.bnd_stage = 1;
+                    Timed t(c->prof, s, K_STAGE1);
+                    hip_check(batch::launch_stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
+                }
+                {
+                    a.src = c->V2;
+                    a.out = nullptr;
+                    a.dose_hi = dhi;
+                    a.dose_lo = dlo;
+                    a.bnd_stage = 2;
+                    Timed t(c->prof, s, K_STAGE2);
+                    hip_check(batch::launch_stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
+                }
+            };
+            if (n_judge == 0) {
+                stages(c->d_list, n_list, c->dose_hi, c->dose_lo);
+            } else {
+                // The judged beamlets take the step twice from the same state:
+                // on every tile of their box (dose into the full pair), then
+                // on the skip set with the other beamlets (dose into the skip
+                // pair, then added to the dose).  State and dose before the
+                // step are saved as the beamlet's pending checkpoint.
+                const uint8_t* d_judge = c->d_sel;
+                const uint8_t* d_save = c->d_sel + nb;
+                hip_check(hipMemcpyAsync(c->d_sel, c->h_sel, 2 * nb, hipMemcpyHostToDevice, s),
+                          "judge flag upload");
+                slices(SLICE_SAVE, d_save);
+                slices(SLICE_RESET, d_judge);
+                BatchMaskArgs fa{c->d_box,   c->d_step, c->d_tile_box, int(T), c->d_tile_max,
+                                 c->d_flags[cur], c->d_flags_full, 0.f, 0, 0, 0, 1,
+                                 nullptr,      nullptr,   nullptr,       d_judge};
+                hip_check(batch::launch_mask(fa, s), "judge tile set");
+                hip_check(batch::launch_compact(c->d_flags_full, int(T), c->d_list_full,
+                                                c->d_count_full, s),
+                          "judge tile list");
+                c->prof.tiles_evaluated += n_full;
+                stages(c->d_list_full, n_full, c->jf_hi, c->jf_lo);
+                slices(SLICE_RESTORE, d_save);
+                BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags_full,
+                                 c->d_flags[cur], c->U, c->V1, c->V2};
+                hip_check(batch::launch_zero(za, s), "judge tile reset");
+                a.judge = d_judge;
+                a.judge_hi = c->js_hi;
+                a.judge_lo = c->js_lo;
+                stages(c->d_list, n_list, c->dose_hi, c->dose_lo);
+                a.judge = nullptr;
+                slices(SLICE_ADD, d_judge);
+                hip_check(hipMemsetAsync(c->d_jdiff, 0, nb * sizeof(unsigned), s),
+                          "judge reset");
+                hip_check(hipMemsetAsync(c->d_jfull, 0, nb * sizeof(unsigned), s),
+                          "judge reset");
+                BatchCompareArgs ca{c->d_box, d_judge,  B,         c->jf_hi,  c->jf_lo,
+                                    c->js_hi, c->js_lo, c->d_jdiff, c->d_jfull};
+                hip_check(batch::launch_compare(ca, int(c->max_box_cells), s), "judge compare");
+                hip_check(hipMemcpyAsync(c->h_jdiff, c->d_jdiff, nb * sizeof(unsigned),
+                                         hipMemcpyDeviceToHost, s),
+                          "judge read-back");
+                hip_check(hipMemcpyAsync(c->h_jfull, c->d_jfull, nb * sizeof(unsigned),
+                                         hipMemcpyDeviceToHost, s),
+                          "judge read-back");
+                sync(c);
             }
 
             // 4. Energies and the next step's tile sets.
             n_active = 0;
+            int n_back = 0;
             for (int b = 0; b < B; ++b) {
+                const size_t k = size_t(b);
+                c->h_step[b].full_set = 0;
+                if (n_judge)
+                    h_back[b] = 0;
                 if (!c->h_step[b].active)
                     continue;
-                E[size_t(b)] -= dE[size_t(b)];
-                steps[size_t(b)] += 1;
-                const bool on = E[size_t(b)] > n.emin;
+                E[k] -= dE[k];
+                steps[k] += 1;
+                since[k] += 1;
+                if (n_judge && h_judge[b]) {
+                    // accepted: the pending checkpoint becomes the last good
+                    // one; rejected: back to the last good checkpoint, on
+                    // every tile for one step, without the network after
+                    float df, fl;
+                    std::memcpy(&df, &c->h_jdiff[b], sizeof df);
+                    std::memcpy(&fl, &c->h_jfull[b], sizeof fl);
+                    c->judged += 1;
+                    if (!(df <= c->judge_tol * fl) && !(fl == 0.f && df == 0.f)) {
+                        c->rejected[k] = 1;
+                        c->rejections += 1;
+                        h_back[b] = uint8_t(acc[k] < 0 ? 3 : 1 + acc[k]);
+                        E[k] = acc[k] < 0 ? E0[k] : ckE[acc[k]][k];
+                        steps[k] = acc[k] < 0 ? 0 : ckS[acc[k]][k];
+                        c->rollback_to[k] = steps[k];
+                        c->h_step[b].full_set = 1;
+                        n_back += 1;
+                    } else {
+                        acc[k] = pend[k];
+                        since[k] = 1;
+                    }
+                }
+                const bool on = E[k] > n.emin;
                 c->h_step[b].active = on ? 1 : 0;
-                c->h_step[b].inflow_on = on ? inflow_on(c->bl[size_t(b)], n, E[size_t(b)]) : 0;
+                c->h_step[b].inflow_on = on ? inflow_on(c->bl[k], n, E[k]) : 0;
+                c->h_step[b].model_on = use_model && !c->rejected[k] ? 1 : 0;
+                c->h_step[b].energy = float(E[k]);
                 n_active += on ? 1 : 0;
+            }
+            if (n_back) {
+                hip_check(hipMemcpyAsync(c->d_sel + 2 * nb, h_back, nb, hipMemcpyHostToDevice, s),
+                          "roll-back flag upload");
+                slices(SLICE_RESTORE, c->d_sel + 2 * nb);
             }
             if (n_active == 0)
                 break;
+            iter += 1;
             hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
                                      hipMemcpyHostToDevice, s),
                       "box flag upload");
+            const float thr = float(n.tile_threshold);
+            if (use_model) {
+                // a tile predicted to stay empty is left out for `horizon` steps
+                const mlp::DeviceParams prm = mlp::device_params_of(c->model);
+                BatchFeatureArgs fa{c->d_box,    c->d_step,    c->d_tile_box,   int(T),
+                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_rho,
+                                    nullptr,     prm.w,        prm.p,           c->decision,
+                                    thr,         c->d_pred,    c->horizon};
+                Timed t(c->prof, s, K_MODEL);
+                hip_check(batch::launch_features(fa, s), "tile-skip network kernel");
+            }
             {
                 BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
                                  int(T),        c->d_tile_max,   c->d_flags[cur],
-                                 c->d_flags[1 - cur], float(n.tile_threshold),
+                                 c->d_flags[1 - cur], thr,
                                  tile_radius(F32_TX), tile_radius(F32_TY), tile_radius(F32_TZ),
-                                 skipping ? 0 : 1};
+                                 skipping ? 0 : 1,
+                                 use_model ? c->d_pred : nullptr,
+                                 c->recording ? c->d_ring : nullptr,
+                                 c->recording ? c->d_wmax : nullptr,
+                                 nullptr};
                 Timed t(c->prof, s, K_TILE_MASK);
                 hip_check(batch::launch_mask(ma, s), "tile mask kernel (batch)");
                 hip_check(batch::launch_compact(c->d_flags[1 - cur], int(T), c->d_list,
@@ -549,6 +827,22 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                  c->d_flags[1 - cur], c->U, c->V1, c->V2};
                 Timed t(c->prof, s, K_TILE_ZERO);
                 hip_check(batch::launch_zero(za, s), "tile zero kernel (batch)");
+            }
+            if (c->recording && iter % c->record_horizon == 0) {
+                // label the ring saved `horizon` steps ago, then save this step's
+                if (iter >= c->record_horizon) {
+                    BatchLabelArgs la{int(T),     c->d_cand, c->d_feat, c->d_wmax, thr,
+                                      c->d_rx,    c->d_ry,   c->d_rcount, c->record_cap};
+                    hip_check(batch::launch_labels(la, s), "label kernel");
+                }
+                BatchFeatureArgs fa{c->d_box,    c->d_step,    c->d_tile_box,   int(T),
+                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_rho,
+                                    c->d_feat,   nullptr,      nullptr,         0.f,
+                                    thr,         nullptr,      0};
+                hip_check(batch::launch_features(fa, s), "feature kernel");
+                hip_check(hipMemcpyAsync(c->d_cand, c->d_ring, T, hipMemcpyDeviceToDevice, s),
+                          "candidate save");
+                hip_check(hipMemsetAsync(c->d_wmax, 0, T * sizeof(float), s), "window reset");
             }
             cur = 1 - cur;
         }
@@ -648,6 +942,99 @@ int m1_batch_host_ms(m1_batch* c, double* setup_ms)
     return guarded(c, [&] {
         if (setup_ms)
             *setup_ms = c->host_ms;
+    });
+}
+
+int m1_batch_set_skip_model(m1_batch* c, const m1_mlp* model, float decision, int32_t horizon,
+                            int32_t judge_every, float judge_tol)
+{
+    return guarded(c, [&] {
+        if (model) {
+            if (!(decision > 0.f && decision < 1.f))
+                throw ValueError{"decision must lie in (0, 1)"};
+            if (horizon < 1 || horizon > 255)
+                throw ValueError{"horizon must lie in [1, 255]"};
+            if (judge_every < 0)
+                throw ValueError{"judge_every must be >= 0 (0: no judge)"};
+            if (!(judge_tol > 0.f))
+                throw ValueError{"judge_tol must be positive"};
+            if (mlp::device_of(model) != c->device)
+                throw ValueError{"the model lives on another device"};
+            hip_check(mlp::finish(model), "model synchronisation");
+        }
+        sync(c);
+        c->model = model;
+        c->decision = decision;
+        c->horizon = horizon;
+        c->judge_every = judge_every;
+        c->judge_tol = judge_tol;
+        c->judged = c->rejections = 0;
+    });
+}
+
+int m1_batch_record(m1_batch* c, int32_t enable, int32_t full_domain, int32_t horizon,
+                    int64_t capacity)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->free_record_buffers();
+        c->recording = false;
+        if (!enable)
+            return;
+        if (horizon < 1)
+            throw ValueError{"horizon must be at least 1"};
+        if (capacity < 1 || capacity > (int64_t(1) << 28))
+            throw ValueError{"capacity must lie in [1, 2^28]"};
+        dev_alloc(&c->d_rx, size_t(capacity) * mlp::NIN, "hipMalloc samples");
+        dev_alloc(&c->d_ry, size_t(capacity), "hipMalloc labels");
+        dev_alloc(&c->d_rcount, 1, "hipMalloc sample count");
+        hip_check(hipMemset(c->d_rcount, 0, sizeof(int)), "sample count reset");
+        c->record_cap = int(capacity);
+        c->record_horizon = horizon;
+        c->record_full = full_domain != 0;
+        c->recording = true;
+    });
+}
+
+int m1_batch_samples(m1_batch* c, float* x, float* y, int64_t capacity, int64_t* count)
+{
+    return guarded(c, [&] {
+        if (!count)
+            throw ValueError{"count must not be NULL"};
+        int n = 0;
+        if (c->d_rcount) {
+            hip_check(hipMemcpyAsync(&n, c->d_rcount, sizeof(int), hipMemcpyDeviceToHost,
+                                     c->stream),
+                      "sample count read-back");
+            sync(c);
+        }
+        n = std::min(n, c->record_cap);
+        *count = n;
+        const int64_t k = std::min<int64_t>(n, capacity);
+        if (k > 0) {
+            if (!x || !y)
+                throw ValueError{"x and y must not be NULL"};
+            hip_check(hipMemcpyAsync(x, c->d_rx, size_t(k) * mlp::NIN * sizeof(float),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "sample download");
+            hip_check(hipMemcpyAsync(y, c->d_ry, size_t(k) * sizeof(float),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "label download");
+            sync(c);
+        }
+    });
+}
+
+int m1_batch_skip_stats(m1_batch* c, int64_t* judged, int64_t* rejections, int32_t* rollback_to)
+{
+    return guarded(c, [&] {
+        if (judged)
+            *judged = c->judged;
+        if (rejections)
+            *rejections = c->rejections;
+        if (rollback_to && !c->rollback_to.empty())
+            std::memcpy(rollback_to, c->rollback_to.data(),
+                        c->rollback_to.size() * sizeof(int32_t));
     });
 }
 

@@ -204,6 +204,9 @@ struct BoxDesc {
 struct BoxStep {
     int active;     // the beamlet is still marching
     int inflow_on;  // z = 0 tiles stay evaluated: the inflow spectrum is on
+    int model_on;   // the tile-skip network may drop tiles of this box
+    float energy;   // energy at the start of the next step, MeV
+    int full_set;   // every tile of the box is evaluated next step
 };
 
 struct BatchStageArgs {
@@ -223,8 +226,14 @@ struct BatchStageArgs {
     int bnd_stage;           // boundary traces of stage 0, 1 or 2
     const StepCoef32* coef;  // per box
     float* tile_max;         // per global tile (nullable)
+    float* tile_sums;        // per global tile: sum phi, sum |J| (nullable)
     int wave_mode;
     int f64_closure;         // face projection, closure, eigenvalues, HLL in double
+    // stage 2 of boxes flagged in judge (nullable) adds its dose to
+    // judge_hi / judge_lo instead of dose_hi / dose_lo
+    const uint8_t* judge;
+    float* judge_hi;
+    float* judge_lo;
 };
 
 struct BatchCflArgs {
@@ -265,6 +274,87 @@ struct BatchMaskArgs {
     float threshold;
     int rx, ry, rz;
     int all_on;               // no skipping: every tile of an active box
+    uint8_t* pred;            // in/out (nullable): steps a tile stays left out
+                              // (set by the network, counted down here)
+    uint8_t* ring;            // out (nullable): tiles the dilation adds (not above threshold)
+    float* wmax;              // in/out (nullable): running max of tile_max
+    const uint8_t* only;      // (nullable) boxes not flagged here get no tile
+};
+
+// Tile features of the tile-skip network (11 per tile):
+//   log10(max phi)/10, log10(mean phi)/10, sum|J|/sum phi, mean density/1000,
+//   E/250, log10(max phi of the 6 face neighbours)/10 (-x +x -y +y -z +z);
+// phi is the normalised state, log10 of 0 is taken as -30.
+struct BatchFeatureArgs {
+    const BoxDesc* box;
+    const BoxStep* step;
+    const int* tile_box;
+    int n_tiles;
+    const uint8_t* computed;
+    const float* tile_max;
+    const float* tile_sums;
+    const float* tile_rho;    // mean density per tile, kg/m^3
+    float* features;          // out (n_tiles, 11) (nullable)
+    // inference (nullable model), on the tiles evaluated in the last step:
+    // pred[t] = horizon when p_empty > decision, else 0; never set for a tile
+    // above the threshold or a z = 0 tile fed by the inflow
+    const uint16_t* w16;
+    const float* params;
+    float decision;
+    float threshold;
+    uint8_t* pred;
+    int horizon;
+};
+
+// Recording: append (features, label) of the ring tiles saved `horizon`
+// steps ago; label 1 when the tile stayed at or below the threshold.
+struct BatchLabelArgs {
+    int n_tiles;
+    const uint8_t* cand;
+    const float* features;
+    const float* wmax;
+    float threshold;
+    float* out_x;
+    float* out_y;
+    int* count;
+    int cap;
+};
+
+// Judge: per box, the largest |full - skip| dose increment and the largest
+// full increment of one step (ordered float bits).
+struct BatchCompareArgs {
+    const BoxDesc* box;
+    const uint8_t* judge;     // boxes to compare
+    int n_boxes;
+    const float* full_hi;
+    const float* full_lo;
+    const float* skip_hi;
+    const float* skip_lo;
+    unsigned* max_diff;
+    unsigned* max_full;
+};
+
+// Per-box slice operations of the judge, on the boxes with sel[b] != 0:
+//   SLICE_SAVE     state and dose -> checkpoint sel[b] - 1
+//   SLICE_RESTORE  checkpoint sel[b] - 1 -> state and dose (sel[b] == 3: zero)
+//   SLICE_RESET    judge doses (full and skip pairs) <- 0
+//   SLICE_ADD      dose pair += skip pair
+enum { SLICE_SAVE = 0, SLICE_RESTORE = 1, SLICE_RESET = 2, SLICE_ADD = 3 };
+struct BatchSliceArgs {
+    const BoxDesc* box;
+    const uint8_t* sel;
+    int n_boxes;
+    int op;
+    float* U;
+    float* dose_hi;
+    float* dose_lo;
+    float* ck_U[2];
+    float* ck_hi[2];
+    float* ck_lo[2];
+    float* full_hi;
+    float* full_lo;
+    float* skip_hi;
+    float* skip_lo;
 };
 
 struct BatchZeroArgs {
@@ -291,6 +381,10 @@ hipError_t launch_zero(const BatchZeroArgs& a, hipStream_t s);
 hipError_t launch_scatter_dose(const float* hi, const float* lo, int nx, int ny, int nz, int x0,
                                int y0, int py, int pz, double* phantom, hipStream_t s);
 hipError_t launch_box_dose(const float* hi, const float* lo, long n, double* out, hipStream_t s);
+hipError_t launch_features(const BatchFeatureArgs& a, hipStream_t s);
+hipError_t launch_labels(const BatchLabelArgs& a, hipStream_t s);
+hipError_t launch_compare(const BatchCompareArgs& a, int max_cells, hipStream_t s);
+hipError_t launch_slices(const BatchSliceArgs& a, int max_cells, hipStream_t s);
 }  // namespace batch
 
 namespace f32 {

@@ -30,6 +30,7 @@ struct TileView {
     const float* bnd;
     const StepCoef32* coef;
     float* tile_max;   // this tile's slot (nullable)
+    float* tile_sums;  // this tile's sum phi, sum |J| (nullable)
     int wave_mode;
     int tx, ty, tz;    // tile coordinates (x: TX planes, y: TY rows, z: TZ cells)
 };
@@ -587,10 +588,10 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     __shared__ float P[4][PY][PZ];
     __shared__ float Fy[4][TY + 1][TZ];
     __shared__ float Fz[4][TY][TZ + 1];
-    __shared__ float wmax[NT / 64];
+    __shared__ float wmax[3][NT / 64];
 
     const int tid = threadIdx.x;
-    float phimax = 0.f;
+    float phimax = 0.f, phisum = 0.f, jsum = 0.f;
     {
         const unsigned* src = reinterpret_cast<const unsigned*>(a.coef);
         unsigned* dst = reinterpret_cast<unsigned*>(&sc);
@@ -798,6 +799,10 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 for (int c = 0; c < 4; ++c)
                     a.U[i + c * N] = u[c];
                 phimax = fmaxf(phimax, u[0]);
+                if (a.tile_sums) {
+                    phisum += u[0];
+                    jsum += sqrtf(fmaf(u[1], u[1], fmaf(u[2], u[2], u[3] * u[3])));
+                }
                 const float q0 = un[0] * S0;
                 const float q1 = fmaxf(a.V1[i] * sc.g1p[m], 0.f) * sc.S1[m];
                 const float q2 = W[2][0] * S2;
@@ -819,16 +824,29 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     }
     if constexpr (KIND == STAGE2) {
         if (a.tile_max) {
-            for (int off = 32; off > 0; off >>= 1)
+            for (int off = 32; off > 0; off >>= 1) {
                 phimax = fmaxf(phimax, __shfl_xor(phimax, off, 64));
-            if ((tid & 63) == 0)
-                wmax[tid >> 6] = phimax;
+                phisum += __shfl_xor(phisum, off, 64);
+                jsum += __shfl_xor(jsum, off, 64);
+            }
+            if ((tid & 63) == 0) {
+                wmax[0][tid >> 6] = phimax;
+                wmax[1][tid >> 6] = phisum;
+                wmax[2][tid >> 6] = jsum;
+            }
             __syncthreads();
             if (tid == 0) {
-                float m = wmax[0];
-                for (int w = 1; w < NT / 64; ++w)
-                    m = fmaxf(m, wmax[w]);
+                float m = wmax[0][0], ps = wmax[1][0], js = wmax[2][0];
+                for (int w = 1; w < NT / 64; ++w) {
+                    m = fmaxf(m, wmax[0][w]);
+                    ps += wmax[1][w];
+                    js += wmax[2][w];
+                }
                 *a.tile_max = m;
+                if (a.tile_sums) {
+                    a.tile_sums[0] = ps;
+                    a.tile_sums[1] = js;
+                }
             }
         }
     }

@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(StageArgs32 a
         return;  // whole tile and its 9-cell neighbourhood are vacuum
     TileView v{a.g,      a.src,       a.U,   a.V1,  a.out,  a.dose_hi,
                a.dose_lo, a.edep,     a.ids, a.bnd, a.coef, a.tile_max ? a.tile_max + tile : nullptr,
-               a.wave_mode, int(blockIdx.z), int(blockIdx.y), int(blockIdx.x)};
+               nullptr,  a.wave_mode, int(blockIdx.z), int(blockIdx.y), int(blockIdx.x)};
     stage_tile<KIND, IFAC, D>(v);
 }
 

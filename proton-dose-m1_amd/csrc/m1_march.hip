@@ -962,6 +962,16 @@ int m1_set_profiling(m1_ctx* ctx, int32_t enable)
 
 int32_t m1_kernel_count(void) { return K_COUNT; }
 
+void m1_tile_shape(int32_t* tx, int32_t* ty, int32_t* tz)
+{
+    if (tx)
+        *tx = m1::F32_TX;
+    if (ty)
+        *ty = m1::F32_TY;
+    if (tz)
+        *tz = m1::F32_TZ;
+}
+
 const char* m1_kernel_name(int32_t kernel)
 {
     return m1::kernel_name(kernel);

@@ -13,6 +13,8 @@
 
 #include <cstdint>
 
+struct m1_mlp;
+
 namespace m1 {
 namespace mlp {
 
@@ -124,6 +126,11 @@ hipError_t launch_weight_grad(const float* x, const float* acts, const float* de
 hipError_t launch_adam(float* p, uint16_t* w16, const float* grad, float* m, float* v,
                        float lr, float beta1, float beta2, float eps, int step, hipStream_t s);
 hipError_t launch_round_bf16(const float* p, uint16_t* w16, hipStream_t s);
+// Device parameters of a model created by m1_mlp_create.
+DeviceParams device_params_of(const m1_mlp* model);
+int device_of(const m1_mlp* model);
+// Waits for the model's queued work (training, parameter uploads).
+hipError_t finish(const m1_mlp* model);
 
 }  // namespace mlp
 }  // namespace m1

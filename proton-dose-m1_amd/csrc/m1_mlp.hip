@@ -279,6 +279,8 @@ m1_mlp::~m1_mlp()
 namespace m1 {
 namespace mlp {
 DeviceParams device_params_of(const m1_mlp* model) { return {model->w16, model->p}; }
+int device_of(const m1_mlp* model) { return model->device; }
+hipError_t finish(const m1_mlp* model) { return hipStreamSynchronize(model->stream); }
 }  // namespace mlp
 }  // namespace m1
 

@@ -85,6 +85,8 @@ SIGNATURES = [
     ("m1_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_kernel_count", C.c_int32, []),
     ("m1_kernel_name", C.c_char_p, [C.c_int32]),
+    ("m1_tile_shape", None, [C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                             C.POINTER(C.c_int32)]),
     ("m1_kernel_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
     ("m1_reset_stats", C.c_int, [C.c_void_p]),
@@ -105,6 +107,12 @@ SIGNATURES = [
                                       C.POINTER(C.c_int64)]),
     ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
+    ("m1_batch_set_skip_model", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
+                                          C.c_int32, C.c_float]),
+    ("m1_batch_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
+    ("m1_batch_samples", C.c_int, [C.c_void_p, _fp, _fp, C.c_int64, C.POINTER(C.c_int64)]),
+    ("m1_batch_skip_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      _ip]),
     ("m1_mlp_param_count", C.c_int32, []),
     ("m1_mlp_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
     ("m1_mlp_destroy", None, [C.c_void_p]),
@@ -349,6 +357,14 @@ class Context:
         return out
 
 
+def tile_shape():
+    """Cells per FP32-bulk workgroup tile (x, y, z) of the loaded library."""
+    lib = load_library()
+    v = [C.c_int32() for _ in range(3)]
+    lib.m1_tile_shape(*(C.byref(x) for x in v))
+    return tuple(x.value for x in v)
+
+
 class Batch:
     """One libm1march batch: a phantom and its beamlet boxes in device memory."""
 
@@ -361,6 +377,7 @@ class Batch:
         self.h = h
         self.shape = None
         self.boxes = ()
+        self.model = None
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -455,6 +472,37 @@ class Batch:
             self._check(self.lib.m1_batch_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
             out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
         return out
+
+    def set_skip_model(self, model, decision=0.5, horizon=8, judge_every=64, judge_tol=1e-3):
+        """Use the tile-skip network `model` (an Mlp, or None to switch it off)."""
+        h = None if model is None else model.h
+        self._check(self.lib.m1_batch_set_skip_model(self.h, h, float(decision), int(horizon),
+                                                     int(judge_every), float(judge_tol)))
+        self.model = model  # the library keeps a pointer: keep the model alive
+
+    def record(self, enable=True, full_domain=True, horizon=8, capacity=1 << 20):
+        """Record (features, label) samples during the following marches."""
+        self._check(self.lib.m1_batch_record(self.h, int(bool(enable)), int(bool(full_domain)),
+                                             int(horizon), int(capacity)))
+
+    def samples(self):
+        """Recorded samples: x (n, 11) float32 and labels y (n,) float32."""
+        n = C.c_int64()
+        self._check(self.lib.m1_batch_samples(self.h, None, None, 0, C.byref(n)))
+        x = np.empty((n.value, Mlp.N_INPUTS), dtype=np.float32)
+        y = np.empty(n.value, dtype=np.float32)
+        if n.value:
+            self._check(self.lib.m1_batch_samples(self.h, _ptr(x), _ptr(y), n.value,
+                                                  C.byref(n)))
+        return x, y
+
+    def skip_stats(self):
+        """(judge checks, rejections, per-beamlet step it was sent back to or -1)."""
+        j = C.c_int64()
+        r = C.c_int64()
+        back = np.full(len(self.boxes), -1, dtype=np.int32)
+        self._check(self.lib.m1_batch_skip_stats(self.h, C.byref(j), C.byref(r), _ptr(back)))
+        return j.value, r.value, back
 
 
 class Mlp:

@@ -136,6 +136,27 @@ def batch_workload(n_beamlets=256, phantom_seed=4, beamlet_seed=40, shape=(200, 
     return phantom, beamlets
 
 
+def batch_workload_config(tile_threshold=1e-10):
+    """Physics and numerics of the batched workloads: Wentzel scattering and
+    nuclear removal, WENO-Z, IF-SSPRK3 with exact M1 speeds, max step
+    0.25 MeV, CFL 0.9, energy_min 1 MeV, FP32-bulk with tile skipping.  Its
+    geometry and beam are placeholders: the batch takes the phantom and the
+    beamlets."""
+    from .beam import Beam as _Beam
+    from .config import DeviceOptions, Numerics, Physics, SolverConfig
+    from .enums import EnergyScheme, Scattering, SpatialScheme, WaveSpeeds
+
+    return SolverConfig(
+        geometry=Geometry(size=(0.2, 0.2, 0.3), resolution=(200, 200, 300)),
+        beam=_Beam(energy=150.0),
+        physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
+        numerics=Numerics(spatial_scheme=SpatialScheme.WENO_Z,
+                          energy_scheme=EnergyScheme.IF_SSPRK3, wave_speeds=WaveSpeeds.EXACT,
+                          max_energy_step=0.25, cfl=0.9, energy_min=1.0),
+        device=DeviceOptions(precision=Precision.FP32_BULK, tile_threshold=tile_threshold),
+    )
+
+
 # ------------------------------------------------------------------ boxes
 def _residual_range_tables(materials, e_lo, e_hi, n=4000):
     """Per material: energies and CSDA residual range R(E) = int_{e_lo}^E dE'/S."""
@@ -192,7 +213,8 @@ def beamlet_boxes(phantom: Phantom, beamlets, energy_min=1.0, window_sigma=5.0,
     """(x0, y0, nx, ny, nz) of each beamlet's box: `lateral` cells centred on
     the cell holding the entry point (clamped into the phantom), depth to the
     conservative range of E + window_sigma * sigma_E, rounded up to a multiple
-    of `z_multiple` cells (the workgroup tile depth) within the phantom."""
+    of `z_multiple` cells (a multiple of the workgroup tile depth) within the
+    phantom."""
     px, py, pz = phantom.shape
     dx, dy, _ = phantom.spacing
     bx, by = (min(int(v), p) for v, p in zip(lateral, (px, py)))

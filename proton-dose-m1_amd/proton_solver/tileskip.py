@@ -1,0 +1,173 @@
+"""Learned tile skipping for the batched march (no reference counterpart).
+
+The batched march evaluates a 16x16x16-cell tile when a tile within the
+9-cell dilation radius holds phi above the tile threshold.  Most tiles the
+dilation adds stay empty: the tile-skip network (11 features per tile, three
+hidden layers of 64 ReLU units, bf16 weights, float32 accumulation; the
+device kernels are in csrc/m1_mlp.hip) predicts which of them stay at or
+below the threshold over the next `horizon` steps, and the march leaves those
+out.  Tiles above the threshold and inflow tiles are always evaluated.  A
+periodic judge takes a beamlet's step both ways from the same state; a
+beamlet whose step dose moves by more than `judge_tol` of the step's largest
+increment is sent back to its last accepted checkpoint and marches on with
+the dilation rule alone (include/m1march.h, m1_batch_set_skip_model).
+
+Training data comes from the march itself: ``record_samples`` runs the
+batched march with every tile of the active boxes evaluated and labels each
+dilation-only tile by whether it stayed at or below the threshold over the
+next `horizon` steps.  ``train`` fits the network on the device.  ``main``
+records on the training workload (its phantom and beamlets are drawn from
+other seeds than the 256-beamlet workload), trains and writes the weights
+that ``load_model`` reads by default.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+WEIGHTS_FILE = Path(__file__).with_name("tileskip_weights.npy")
+
+# Feature order of the device kernel (include/m1march.h).
+FEATURES = ("log10_max_phi", "log10_mean_phi", "flux_ratio", "density", "energy",
+            "nbr_mx", "nbr_px", "nbr_my", "nbr_py", "nbr_mz", "nbr_pz")
+
+
+@dataclass(frozen=True)
+class SkipOptions:
+    decision: float = 0.9      # probability of staying empty needed to leave a tile out
+    horizon: int = 8           # steps a tile is left out for (the label's window)
+    judge_every: int = 32      # steps between judged steps of a beamlet
+    judge_tol: float = 1e-3    # allowed step-dose change over the step's largest increment
+
+
+def record_samples(handle: _native.Batch, num, horizon=8, capacity=1 << 22,
+                   full_domain=True):
+    """Run one march of a prepared batch with recording on; returns x (n, 11)
+    float32 features and y (n,) float32 labels (1: the tile stayed empty)."""
+    handle.set_skip_model(None)
+    handle.record(True, full_domain=full_domain, horizon=horizon, capacity=capacity)
+    try:
+        handle.march(num, want_dose=False)
+        return handle.samples()
+    finally:
+        handle.record(False)
+
+
+def init_params(seed=0):
+    """He-initialised float32 parameters, biases zero."""
+    rng = np.random.default_rng(seed)
+    n_in, n_h = len(FEATURES), 64
+    parts = []
+    for fan_in, shape in ((n_in, (n_in, n_h)), (None, (n_h,)), (n_h, (n_h, n_h)), (None, (n_h,)),
+                          (n_h, (n_h, n_h)), (None, (n_h,)), (n_h, (n_h,)), (None, (1,))):
+        if fan_in is None:
+            parts.append(np.zeros(int(np.prod(shape))))
+        else:
+            parts.append(rng.normal(0.0, np.sqrt(2.0 / fan_in), size=shape).ravel())
+    return np.concatenate(parts).astype(np.float32)
+
+
+def train(x, y, device=0, epochs=20, batch=4096, lr=1e-3, seed=0, params=None):
+    """Fit a network on the device (Adam on the mean binary cross-entropy).
+    Returns (model, final loss)."""
+    model = _native.Mlp(device, init_params(seed) if params is None else params)
+    loss = model.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return model, loss
+
+
+def confusion(model: _native.Mlp, x, y, decision):
+    """Counts on (x, y): tiles left out that did not stay empty (false_skip)
+    and tiles left out that did (true_skip)."""
+    p = model.forward(x)
+    skip = p > decision
+    empty = y > 0.5
+    return {"samples": int(len(y)), "empty_fraction": float(empty.mean()) if len(y) else 0.0,
+            "true_skip": int((skip & empty).sum()), "false_skip": int((skip & ~empty).sum()),
+            "kept": int((~skip).sum())}
+
+
+def save_params(params, path=WEIGHTS_FILE):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    np.save(Path(path), p, allow_pickle=False)
+
+
+def load_params(path=WEIGHTS_FILE):
+    p = np.load(Path(path), allow_pickle=False)
+    n = _native.load_library().m1_mlp_param_count()
+    if p.dtype != np.float32 or p.shape != (n,):
+        raise ValueError(f"{path}: expected {n} float32 parameters, got {p.dtype} {p.shape}")
+    return p
+
+
+def load_model(device=0, path=WEIGHTS_FILE) -> _native.Mlp:
+    """The network with the weights written by ``main``."""
+    if not Path(path).exists():
+        raise FileNotFoundError(f"{path}: no tile-skip weights; run "
+                                "python -m proton_solver.tileskip to train them")
+    return _native.Mlp(device, load_params(path))
+
+
+def enable(handle: _native.Batch, model, options: SkipOptions = SkipOptions()):
+    """Use `model` (None: switch the network off) in the marches of `handle`."""
+    handle.set_skip_model(model, options.decision, options.horizon, options.judge_every,
+                          options.judge_tol)
+
+
+def training_workload(n_beamlets=96):
+    """Phantom and beamlets of the training runs: the 256-beamlet workload's
+    construction with other seeds (phantom 11, beamlets 110)."""
+    from .batch import batch_workload
+
+    return batch_workload(n_beamlets=n_beamlets, phantom_seed=11, beamlet_seed=110)
+
+
+def main(argv=None):
+    from .batch import batch_workload_config, prepare_batch
+
+    ap = argparse.ArgumentParser(description="record samples, train the tile-skip network "
+                                             "and write its weights")
+    ap.add_argument("--beamlets", type=int, default=96)
+    ap.add_argument("--horizon", type=int, default=SkipOptions.horizon)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--decision", type=float, default=SkipOptions.decision)
+    ap.add_argument("--out", type=Path, default=WEIGHTS_FILE)
+    args = ap.parse_args(argv)
+
+    cfg = batch_workload_config()
+    phantom, beamlets = training_workload(args.beamlets)
+    handle, _ = prepare_batch(phantom, beamlets, cfg)
+    num = _native.numerics_struct(cfg)
+    t0 = time.perf_counter()
+    x, y = record_samples(handle, num, horizon=args.horizon)
+    rec_s = time.perf_counter() - t0
+    print(json.dumps({"recorded": len(y), "empty_fraction": float(y.mean()),
+                      "seconds": round(rec_s, 1)}), flush=True)
+    rng = np.random.default_rng(args.seed)
+    order = rng.permutation(len(y))
+    n_test = len(y) // 10
+    test, fit = order[:n_test], order[n_test:]
+    t0 = time.perf_counter()
+    model, loss = train(x[fit], y[fit], epochs=args.epochs, batch=args.batch, lr=args.lr,
+                        seed=args.seed)
+    report = {"train_loss": loss, "train_seconds": round(time.perf_counter() - t0, 1),
+              "held_out": confusion(model, x[test], y[test], args.decision)}
+    save_params(model.get_params(), args.out)
+    report["weights"] = str(args.out)
+    print(json.dumps(report), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
