@@ -54,6 +54,11 @@ STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
 #   stage 1: read V1 (16) + U (16) + ids (1), write V2 (16)
 #   stage 2: read V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16), dose pair RMW (16)
 STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 69}
+# HBM bytes per evaluated cell of the batched stage kernels, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes scaled by the same counters on a kernel of
+# known traffic (profiles/stage_kernels_batch_pmc_traffic.md); None until
+# measured for this build.
+PMC_BYTES_PER_CELL_FP32 = None
 
 
 def workload_config(precision):
@@ -111,8 +116,9 @@ def cpu_baseline(cfg, n_steps_full, sample_steps):
     }
 
 
-def roofline_of(stats, stage_bytes, cells_per_launch):
-    """HBM roofline of the stage kernel with the largest total time."""
+def roofline_of(stats, stage_bytes, cells_per_launch, traffic_per_cell=None):
+    """HBM roofline of the stage kernel with the largest total time; traffic
+    is the measured HBM bytes per launch when per-cell figures are given."""
     best = None
     for name, per_cell in stage_bytes.items():
         launches, total_ms = stats.get(name, (0, 0.0))
@@ -127,7 +133,8 @@ def roofline_of(stats, stage_bytes, cells_per_launch):
     name, total_ms, achieved, avg_s, per_cell, cells = best
     return {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+        "frac": round(achieved / PEAK_HBM_GBS, 5),
+        "traffic": (round(traffic_per_cell[name] * cells) if traffic_per_cell else None),
         "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
         "algorithmic_bytes_per_cell": per_cell, "cells_per_launch": round(cells),
     }
@@ -214,7 +221,8 @@ def run_batch(args):
         net_doses = [handle.beamlet_dose(b) for b in range(B)]
         beamlet_dose = net_doses.__getitem__
         net = {"weights": tileskip.WEIGHTS_FILE.name, "decision": net_opts.decision,
-               "horizon": net_opts.horizon, "judge_every": net_opts.judge_every,
+               "horizon": net_opts.horizon, "infer_every": net_opts.infer_every,
+               "judge_every": net_opts.judge_every,
                "judge_tol": net_opts.judge_tol, "judged_steps": judged,
                "rejected_beamlets": rejections}
         if not args.no_network_reference:
@@ -247,14 +255,22 @@ def run_batch(args):
     def cells_per_launch(launches):
         return ev * float(tx * ty * tz) * fill / launches
 
-    roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch)
+    roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch, PMC_BYTES_PER_CELL_FP32)
+    if args.profile_calibration:
+        # a kernel of known traffic for scaling the HBM counters: the beamlet
+        # dose read-out reads a float pair and writes a double per box cell
+        x0, y0, nx, ny, nz = boxes[0]
+        handle.beamlet_dose(0)
+        print(json.dumps({"calibration_kernel": "box_dose_kernel", "cells": nx * ny * nz,
+                          "bytes_read": 8 * nx * ny * nz, "bytes_written": 8 * nx * ny * nz}),
+              file=sys.stderr, flush=True)
 
     # Plain FP64 full-domain march of a sample of the beamlets (outside the timed region).
     fp64 = None
     if args.fp64_sample > 0:
         order = np.argsort([b.energy for b in beamlets])
         picks = [int(order[int(round(q * (B - 1)))])
-                 for q in np.linspace(0.0, 0.5, args.fp64_sample)]
+                 for q in np.linspace(0.0, 1.0, args.fp64_sample)]
         ctx = _native.Context(0)
         num64 = _native.numerics_struct(cfg, _native.FP64)
         rows = []
@@ -290,9 +306,8 @@ def run_batch(args):
                          "(bitwise the CPU oracle)",
             "samples": rows,
             "fp64_beamlets_per_s": 1.0 / mean_s,
-            "note": "beamlets at the 0th and 50th energy percentiles of the batch; the FP64 "
-                    "time of the mean beamlet is at least the sample mean (steps grow with "
-                    "energy), so the speed-up is a lower bound",
+            "note": "beamlets at evenly spaced energy ranks of the batch, lowest to highest; "
+                    "the FP64 beamlets/s is the inverse of the sample's mean time",
             "worst_max_dose_rel_err": max(abs(r["max_dose_rel_err"]) for r in rows),
             "min_gamma_pass_percent": min(r["gamma_1pct_1mm_pass_percent"] for r in rows),
         }
@@ -355,12 +370,14 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--beamlets", type=int, default=256)
-    ap.add_argument("--fp64-sample", type=int, default=2)
+    ap.add_argument("--fp64-sample", type=int, default=3)
     ap.add_argument("--tile-threshold", type=float, default=1e-10)
     ap.add_argument("--skip-network", default="auto", choices=["auto", "on", "off"],
                     help="tile-skip network in the batched march (auto: when its weights exist)")
     ap.add_argument("--no-network-reference", action="store_true",
                     help="skip the untimed march without the network")
+    ap.add_argument("--profile-calibration", action="store_true",
+                    help="run a kernel of known HBM traffic after the timed march")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

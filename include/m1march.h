@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define M1_ABI_VERSION 3
+#define M1_ABI_VERSION 4
 
 enum {
     M1_OK = 0,
@@ -112,6 +112,20 @@ typedef struct {
      * tile.  No reference counterpart (the reference's closest analogue is
      * its moving active domain, moving_active_domain.py:107-193). */
     double tile_threshold;
+    /* Batched march with tile_threshold > 0 only (ignored elsewhere, as the
+     * reference's static energy_march ignores it): the vacuum-aware CFL of
+     * the reference's moving active domain (energy.py:459-537, 610-636,
+     * config.py:49).  A cell's axis speed counts only within 3 cells along
+     * that axis of an active cell, active meaning normalised phi (or the z = 0
+     * inflow phi) above tile_threshold; the reference's relative tolerance
+     * is taken as the tile threshold, relative to the beamlet's peak inflow
+     * fluence instead of the current maximum. */
+    int32_t vacuum_aware_cfl;
+    /* Batched march only: 0 bounds a beamlet's CFL step, as the march of
+     * that beamlet on the whole phantom does, by the phantom outside its box
+     * too (held at the zero state there; not with vacuum_aware_cfl, which
+     * drops vacuum far from active cells); 1 bounds the box alone. */
+    int32_t batch_cfl_box_only;
 } m1_numerics;
 
 /* Library ABI version (M1_ABI_VERSION). */
@@ -292,14 +306,15 @@ int m1_mlp_train(m1_mlp *model, const float *x, const float *y, int32_t n,
  *                                         face neighbours (-x +x -y +y -z +z)
  * (phi below 1e-30 counts as 1e-30; tiles not evaluated count as empty).
  *
- * Label: 1 when a tile that only the dilation adds to the set (its own phi
- * is below the tile threshold, it is not an inflow tile) keeps phi at or
- * below the threshold over the next `horizon` steps.
+ * Label: 1 when a candidate tile (below) keeps phi at or below the
+ * threshold over the next `horizon` steps.
  *
- * With a model set, the network is evaluated after every step on the tiles
- * that step evaluated; a tile whose probability exceeds `decision` is left
- * out for the next `horizon` steps (1..255) whenever only the dilation would
- * add it.  Tiles above the threshold and inflow tiles are never left out.
+ * With a model set, the network is evaluated every `infer_every` steps
+ * (1..horizon) on the tiles that step evaluated; a tile whose probability
+ * exceeds `decision` is left out for the next `horizon` steps (1..255)
+ * whenever it is a candidate: the reach rule adds it while no cell within
+ * 4 cells of it holds phi above the threshold (checked every step).  Tiles
+ * above the threshold and inflow tiles are never left out.
  * Every `judge_every` (0: never)
  * steps of a beamlet, and on its last step, the judge takes that step twice
  * from the same state: on every tile of the beamlet's box and on the tile set
@@ -312,7 +327,8 @@ int m1_mlp_train(m1_mlp *model, const float *x, const float *y, int32_t n,
  * ------------------------------------------------------------------ */
 /* model NULL switches the network off. */
 int m1_batch_set_skip_model(m1_batch *batch, const m1_mlp *model, float decision,
-                            int32_t horizon, int32_t judge_every, float judge_tol);
+                            int32_t horizon, int32_t infer_every, int32_t judge_every,
+                            float judge_tol);
 /* Record (features, label) samples of the dilation-only tiles during the
  * following marches, every `horizon` steps, up to `capacity` samples.
  * full_domain: evaluate every tile of the active boxes (labels from the

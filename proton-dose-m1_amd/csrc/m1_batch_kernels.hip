@@ -67,11 +67,84 @@ __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArg
                a.coef + b,
                a.tile_max ? a.tile_max + t : nullptr,
                a.tile_sums ? a.tile_sums + 2 * long(t) : nullptr,
+               a.tile_face ? a.tile_face + FACE_SLOTS * long(t) : nullptr,
                a.wave_mode,
                tc.tx,
                tc.ty,
                tc.tz};
     stage_tile<KIND, IFAC, D>(v);
+}
+
+// Vacuum-aware CFL of one live tile (energy.py:459-537, 610-636): cell
+// activity over the tile and a 3-cell halo along each axis (cells of tiles
+// outside the live set hold the zero state and are inactive), then each
+// cell's axis speed counts where an active cell lies within 3 cells along
+// that axis.  Tiles outside the live set contribute nothing: an active cell
+// within 3 cells of one would have kept it in the set, whose reach is 9.
+constexpr int CFL_HALO = 3;
+constexpr int AX = TX + 2 * CFL_HALO, AY = TY + 2 * CFL_HALO, AZ = TZ + 2 * CFL_HALO;
+
+__device__ void cfl_tile_vacuum_aware(const BatchCflArgs& a, const BoxDesc& d,
+                                      const TileCoord& tc, int b, double* red)
+{
+    __shared__ uint8_t act[AX][AY][AZ];
+    const StepCoef& c = a.coef[b];
+    const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
+    const long N = long(d.nx) * d.ny * d.nz;
+    const double pin_scale = c.ed[0] * c.inv_alpha, rf = c.rf;
+    const uint8_t* ids = a.ids + d.cell_off;
+    const double* spatial = a.spatial + d.col_off;
+    const float* U = a.U + 4 * d.cell_off;
+    const int tid = threadIdx.x;
+    const int x0 = tc.tx * TX, y0 = tc.ty * TY, z0 = tc.tz * TZ;
+    for (int k = tid; k < AX * AY * AZ; k += NT) {
+        const int lz = k % AZ, ly = (k / AZ) % AY, lx = k / (AZ * AY);
+        const int out = (lx < CFL_HALO || lx >= TX + CFL_HALO) + (ly < CFL_HALO || ly >= TY + CFL_HALO) +
+                        (lz < CFL_HALO || lz >= TZ + CFL_HALO);
+        const int x = x0 + lx - CFL_HALO, y = y0 + ly - CFL_HALO, z = z0 + lz - CFL_HALO;
+        bool on = false;
+        if (out <= 1 && x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz) {
+            const int ux = tc.tx + (lx < CFL_HALO ? -1 : (lx >= TX + CFL_HALO ? 1 : 0));
+            const int uy = tc.ty + (ly < CFL_HALO ? -1 : (ly >= TY + CFL_HALO ? 1 : 0));
+            const int uz = tc.tz + (lz < CFL_HALO ? -1 : (lz >= TZ + CFL_HALO ? 1 : 0));
+            const int u = d.tile_off + (ux * d.nty + uy) * d.ntz + uz;
+            const long i = (long(x) * d.ny + y) * d.nz + z;
+            if (a.live[u])
+                on = U[i] > a.act_threshold;
+            if (z == 0)
+                on = on || spatial[long(x) * d.ny + y] * pin_scale > double(a.act_threshold);
+        }
+        act[lx][ly][lz] = on ? 1 : 0;
+    }
+    __syncthreads();
+    const double eps = c.eig_eps;
+    const int tz = tid % TZ, ty = tid / TZ;
+    const int z = z0 + tz, y = y0 + ty;
+    const int x1 = min(x0 + TX, d.nx);
+    double mx = 0.0;
+    bool bad = false;
+    if (z < d.nz && y < d.ny) {
+        for (int x = x0; x < x1; ++x) {
+            const int lx = x - x0 + CFL_HALO, ly = ty + CFL_HALO, lz = tz + CFL_HALO;
+            bool mxk = false, myk = false, mzk = false;
+#pragma unroll
+            for (int q = -CFL_HALO; q <= CFL_HALO; ++q) {
+                mxk = mxk || act[lx + q][ly][lz];
+                myk = myk || act[lx][ly + q][lz];
+                mzk = mzk || act[lx][ly][lz + q];
+            }
+            const long i = (long(x) * d.ny + y) * d.nz + z;
+            const bool zmin = z == 0;
+            const double pin = zmin ? spatial[i / d.nz] * pin_scale : 0.0;
+            const double v = cfl_cell_masked(U, i, N, c.S0[ids[i]], g, eps, zmin, pin, rf, mxk,
+                                             myk, mzk);
+            if (!(v == v))
+                bad = true;
+            else
+                mx = fmax(mx, v);
+        }
+    }
+    block_max_to(mx, bad, red, a.out + b);
 }
 
 // CFL maximum per box over all its cells.  Tiles outside the live set hold
@@ -84,6 +157,13 @@ __global__ void __launch_bounds__(NT) cfl_kernel(BatchCflArgs a)
     const int b = a.tile_box[t];
     if (!a.step[b].active)
         return;
+    if (a.vacuum_aware) {
+        if (!a.live[t])
+            return;
+        const BoxDesc d = a.box[b];
+        cfl_tile_vacuum_aware(a, d, tile_coord(d, b, t), b, red);
+        return;
+    }
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
     const StepCoef& c = a.coef[b];
@@ -145,12 +225,13 @@ __global__ void boundary_kernel(BatchBoundaryArgs a)
 }
 
 // Next-step tile set of every active box: the single-box rule (a tile is
-// evaluated when a tile within the dilation radius holds phi above the
-// threshold, or it is a z = 0 tile while the inflow is on) per box.  The
-// tile-skip network may leave out, for the steps its countdown runs, tiles
-// that only the dilation adds (never a tile above the threshold or one fed
-// by the inflow).  Also records, when asked, the tiles the dilation adds
-// (ring) and the running max of tile_max.
+// evaluated when it holds phi above the threshold, when a cell of a
+// neighbouring tile within TILE_REACH cells of it does, or when it is a z = 0
+// tile while the inflow is on) per box.  The tile-skip network may leave out,
+// for the steps its countdown runs, tiles the reach rule adds while no cell
+// within TILE_GUARD cells of them holds phi above the threshold (checked
+// every step).  Also records, when asked, those candidate tiles (ring) and
+// the running max of tile_max.
 __global__ void mask_kernel(BatchMaskArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -170,7 +251,8 @@ __global__ void mask_kernel(BatchMaskArgs a)
     if (a.wmax)
         a.wmax[t] = fmaxf(a.wmax[t], self_max);
     const bool forced = st.inflow_on && tc.tz == 0;
-    bool on = forced;
+    const bool self_on = self_max > a.threshold;
+    bool on = forced || self_on;
     for (int dx = -a.rx; dx <= a.rx && !on; ++dx) {
         const int xx = tc.tx + dx;
         if (xx < 0 || xx >= d.ntx)
@@ -181,18 +263,43 @@ __global__ void mask_kernel(BatchMaskArgs a)
                 continue;
             for (int dz = -a.rz; dz <= a.rz && !on; ++dz) {
                 const int zz = tc.tz + dz;
-                if (zz < 0 || zz >= d.ntz)
+                if (zz < 0 || zz >= d.ntz || (dx == 0 && dy == 0 && dz == 0))
                     continue;
                 const int u = d.tile_off + (xx * d.nty + yy) * d.ntz + zz;
-                on = a.computed[u] && a.tile_max[u] > a.threshold;
+                on = a.computed[u] && facing_bound(a.tile_face + FACE_SLOTS * long(u), dx, dy, dz,
+                                                   0) > a.threshold;
             }
         }
     }
-    const bool self_on = self_max > a.threshold;
+    // Candidates of the network: added by the reach rule alone, with nothing
+    // above the threshold within TILE_GUARD cells.
+    bool cand = false;
+    if (on && !forced && !self_on && (a.ring || a.pred)) {
+        bool near = false;
+        for (int dx = -1; dx <= 1 && !near; ++dx) {
+            const int xx = tc.tx + dx;
+            if (xx < 0 || xx >= d.ntx)
+                continue;
+            for (int dy = -1; dy <= 1 && !near; ++dy) {
+                const int yy = tc.ty + dy;
+                if (yy < 0 || yy >= d.nty)
+                    continue;
+                for (int dz = -1; dz <= 1 && !near; ++dz) {
+                    const int zz = tc.tz + dz;
+                    if (zz < 0 || zz >= d.ntz || (dx == 0 && dy == 0 && dz == 0))
+                        continue;
+                    const int u = d.tile_off + (xx * d.nty + yy) * d.ntz + zz;
+                    near = a.computed[u] && facing_bound(a.tile_face + FACE_SLOTS * long(u), dx,
+                                                         dy, dz, 6) > a.threshold;
+                }
+            }
+        }
+        cand = !near;
+    }
     if (a.ring)
-        a.ring[t] = (on && !forced && !self_on) ? 1 : 0;
+        a.ring[t] = cand ? 1 : 0;
     if (a.pred && a.pred[t] > 0) {
-        if (on && !forced && !self_on && st.model_on)
+        if (cand && st.model_on)
             on = false;
         a.pred[t] -= 1;
     }
@@ -232,9 +339,15 @@ __global__ void __launch_bounds__(mlp::WG) feature_kernel(BatchFeatureArgs a)
     const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
+        // the face neighbour's slab within TILE_REACH cells of this tile
         const int xx = tc.tx + nb[k][0], yy = tc.ty + nb[k][1], zz = tc.tz + nb[k][2];
         const bool in = xx >= 0 && xx < d.ntx && yy >= 0 && yy < d.nty && zz >= 0 && zz < d.ntz;
-        x[5 + k] = in ? lg(tmax(d.tile_off + (xx * d.nty + yy) * d.ntz + zz)) : lg(0.f);
+        const int u = in ? d.tile_off + (xx * d.nty + yy) * d.ntz + zz : 0;
+        const float v = in && a.computed[u]
+                            ? facing_bound(a.tile_face + FACE_SLOTS * long(u), nb[k][0], nb[k][1],
+                                           nb[k][2], 0)
+                            : 0.f;
+        x[5 + k] = lg(v);
     }
     if (a.features && st.active)
 #pragma unroll

@@ -50,6 +50,7 @@ struct m1_batch {
     std::vector<m1_beamlet> bl;
     std::vector<BoxDesc> boxes;
     std::vector<double> alpha;
+    std::vector<uint32_t> outside_mats;  // per box: materials of the phantom outside it
     long total_cells = 0, total_cols = 0, max_box_cells = 0;
     int total_tiles = 0, max_cols = 0;
 
@@ -58,7 +59,7 @@ struct m1_batch {
     int* d_tile_box = nullptr;
     uint8_t* d_tile_mats = nullptr;
     float *U = nullptr, *V1 = nullptr, *V2 = nullptr, *dose_hi = nullptr, *dose_lo = nullptr,
-          *bnd = nullptr, *d_tile_max = nullptr;
+          *bnd = nullptr, *d_tile_max = nullptr, *d_tile_face = nullptr;
     uint8_t* ids = nullptr;
     double* spatial = nullptr;
     uint8_t* d_flags[2] = {nullptr, nullptr};
@@ -88,7 +89,7 @@ struct m1_batch {
     // skip network and its judge
     const m1_mlp* model = nullptr;
     float decision = 0.5f, judge_tol = 1e-3f;
-    int horizon = 8, judge_every = 64;
+    int horizon = 8, judge_every = 64, infer_every = 4;
     std::vector<uint8_t> rejected;
     std::vector<int32_t> rollback_to;  // step a rejected beamlet restarted from (-1: none)
     // two checkpoints per box (state and dose before a judged step)
@@ -159,7 +160,7 @@ void m1_batch::free_problem()
     d_tile_rho = d_tile_sums = d_wmax = d_feat = nullptr;
     d_pred = d_ring = d_cand = nullptr;
     void* dev[] = {d_box,  d_step,   d_tile_box, d_tile_mats, U,         V1,       V2,
-                   dose_hi, dose_lo, bnd,        d_tile_max,  ids,       spatial,  d_flags[0],
+                   dose_hi, dose_lo, bnd,        d_tile_max,  d_tile_face, ids,       spatial,  d_flags[0],
                    d_flags[1], d_list, d_count,  d_coef,      d_cfl,     d_coef32, d_red,
                    d_phantom, d_box_dose};
     for (void* p : dev)
@@ -173,7 +174,7 @@ void m1_batch::free_problem()
     d_step = nullptr;
     d_tile_box = nullptr;
     d_tile_mats = nullptr;
-    U = V1 = V2 = dose_hi = dose_lo = bnd = d_tile_max = nullptr;
+    U = V1 = V2 = dose_hi = dose_lo = bnd = d_tile_max = d_tile_face = nullptr;
     ids = nullptr;
     spatial = nullptr;
     d_flags[0] = d_flags[1] = nullptr;
@@ -359,15 +360,23 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         std::vector<uint8_t> tile_mats(static_cast<size_t>(tiles), 0);
         std::vector<float> tile_rho(static_cast<size_t>(tiles), 0.f);
         std::vector<int> tile_cells(static_cast<size_t>(tiles), 0);
+        // Materials of the phantom outside each box: the whole-phantom march
+        // holds them at the zero state, whose CFL value bounds the step.
+        std::vector<long> phantom_count(MAXMAT, 0);
+        for (long i = 0; i < long(nx) * ny * nz; ++i)
+            phantom_count[size_t(ids[i])] += 1;
+        std::vector<uint32_t> outside_mats(size_t(n_beamlets), 0);
         for (int b = 0; b < n_beamlets; ++b) {
             const m1_beamlet& q = beamlets[b];
             const BoxDesc& d = boxes[size_t(b)];
+            std::vector<long> box_count(MAXMAT, 0);
             for (int x = 0; x < q.nx; ++x)
                 for (int y = 0; y < q.ny; ++y) {
                     const int32_t* src = ids + (long(q.x0 + x) * ny + (q.y0 + y)) * nz;
                     uint8_t* dst = box_ids.data() + d.cell_off + (long(x) * q.ny + y) * q.nz;
                     for (int z = 0; z < q.nz; ++z) {
                         dst[z] = uint8_t(src[z]);
+                        box_count[size_t(src[z])] += 1;
                         const int t = d.tile_off +
                                       ((x / F32_TX) * d.nty + y / F32_TY) * d.ntz + z / F32_TZ;
                         tile_mats[size_t(t)] |= uint8_t(1u << src[z]);
@@ -377,6 +386,9 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
                 }
             for (int t = 0; t < d.ntx * d.nty * d.ntz; ++t)
                 tile_box[size_t(d.tile_off + t)] = b;
+            for (int m = 0; m < MAXMAT; ++m)
+                if (phantom_count[size_t(m)] > box_count[size_t(m)])
+                    outside_mats[size_t(b)] |= 1u << m;
         }
         for (size_t t = 0; t < tile_rho.size(); ++t) {
             if (tile_cells[t] > 0)
@@ -395,6 +407,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         c->bl.assign(beamlets, beamlets + n_beamlets);
         c->boxes = boxes;
         c->alpha = alpha;
+        c->outside_mats = std::move(outside_mats);
         c->total_cells = cells;
         c->total_cols = cols;
         c->total_tiles = int(tiles);
@@ -412,6 +425,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         dev_alloc(&c->dose_lo, N, "hipMalloc batch dose");
         dev_alloc(&c->bnd, 12 * size_t(cols), "hipMalloc batch boundary");
         dev_alloc(&c->d_tile_max, T, "hipMalloc tile max");
+        dev_alloc(&c->d_tile_face, T * FACE_SLOTS, "hipMalloc tile face slabs");
         dev_alloc(&c->ids, N, "hipMalloc batch ids");
         dev_alloc(&c->spatial, size_t(cols), "hipMalloc batch inflow");
         dev_alloc(&c->d_flags[0], T, "hipMalloc tile set");
@@ -556,7 +570,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         {
             // Boxes with nothing to march leave the list at once.
             BatchMaskArgs ma{c->d_box,      c->d_step, c->d_tile_box, int(T), c->d_tile_max,
-                             c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1,
+                             c->d_tile_face, c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1,
                              nullptr,       nullptr,   nullptr,   nullptr};
             hip_check(batch::launch_mask(ma, s), "tile mask kernel");
             cur = 1 - cur;
@@ -564,6 +578,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                       "tile compaction kernel");
         }
         const size_t nm = c->mats.size();
+        // the vacuum-aware CFL needs the tile threshold (its activity level)
+        // and the exact speeds (the reference applies no masks to UNIT speeds)
+        const bool vacuum_aware =
+            n.vacuum_aware_cfl && n.tile_threshold > 0.0 && n.wave == M1_WAVE_EXACT;
         const bool skipping = n.tile_threshold > 0.0 && !(c->recording && c->record_full);
         int64_t iter = 0;
 
@@ -584,7 +602,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             {
                 BatchCflArgs a{c->d_box,  c->d_step,        c->d_tile_box, int(T), c->dx,
                                c->dy,     c->dz,            c->U,          c->ids, c->spatial,
-                               c->d_flags[cur], c->d_tile_mats, c->d_cfl,  c->d_red, n.wave};
+                               c->d_flags[cur], c->d_tile_mats, c->d_cfl,  c->d_red, n.wave,
+                               vacuum_aware ? 1 : 0, float(n.tile_threshold)};
                 Timed t(c->prof, s, K_CFL);
                 hip_check(batch::launch_cfl(a, s), "CFL kernel (batch)");
             }
@@ -605,6 +624,19 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     continue;
                 double mx;
                 std::memcpy(&mx, &c->h_red[b], sizeof mx);
+                if (!vacuum_aware && !n.batch_cfl_box_only) {
+                    // the phantom outside the box, at the zero state in the
+                    // whole-phantom march: isotropic speeds (wave_speeds.py:37-50)
+                    const double iso = n.wave == M1_WAVE_UNIT ? 1.0 : 0.57735026918962576;
+                    for (int m = 0; m < int(nm); ++m)
+                        if (c->outside_mats[size_t(b)] >> m & 1u) {
+                            const double s0 = c->h_cfl[b].S0[m];
+                            const double v = iso / (s0 * c->dx) + iso / (s0 * c->dy) +
+                                             iso / (s0 * c->dz);
+                            if (v > mx)
+                                mx = v;
+                        }
+                }
                 double d = cfl_energy_step(n, mx);
                 if (E[size_t(b)] - n.emin < d)
                     d = E[size_t(b)] - n.emin;
@@ -662,6 +694,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             a.bnd = c->bnd;
             a.coef = c->d_coef32;
             a.tile_max = skipping || c->recording ? c->d_tile_max : nullptr;
+            a.tile_face = skipping || c->recording ? c->d_tile_face : nullptr;
             a.tile_sums = use_model || c->recording ? c->d_tile_sums : nullptr;
             a.wave_mode = n.wave;
             a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
@@ -708,7 +741,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 slices(SLICE_SAVE, d_save);
                 slices(SLICE_RESET, d_judge);
                 BatchMaskArgs fa{c->d_box,   c->d_step, c->d_tile_box, int(T), c->d_tile_max,
-                                 c->d_flags[cur], c->d_flags_full, 0.f, 0, 0, 0, 1,
+                                 c->d_tile_face, c->d_flags[cur], c->d_flags_full, 0.f, 0, 0, 0, 1,
                                  nullptr,      nullptr,   nullptr,       d_judge};
                 hip_check(batch::launch_mask(fa, s), "judge tile set");
                 hip_check(batch::launch_compact(c->d_flags_full, int(T), c->d_list_full,
@@ -796,11 +829,12 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                      hipMemcpyHostToDevice, s),
                       "box flag upload");
             const float thr = float(n.tile_threshold);
-            if (use_model) {
+            if (use_model && iter % c->infer_every == 0) {
                 // a tile predicted to stay empty is left out for `horizon` steps
                 const mlp::DeviceParams prm = mlp::device_params_of(c->model);
                 BatchFeatureArgs fa{c->d_box,    c->d_step,    c->d_tile_box,   int(T),
-                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_rho,
+                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_face,
+                                    c->d_tile_rho,
                                     nullptr,     prm.w,        prm.p,           c->decision,
                                     thr,         c->d_pred,    c->horizon};
                 Timed t(c->prof, s, K_MODEL);
@@ -808,7 +842,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             }
             {
                 BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
-                                 int(T),        c->d_tile_max,   c->d_flags[cur],
+                                 int(T),        c->d_tile_max,   c->d_tile_face, c->d_flags[cur],
                                  c->d_flags[1 - cur], thr,
                                  tile_radius(F32_TX), tile_radius(F32_TY), tile_radius(F32_TZ),
                                  skipping ? 0 : 1,
@@ -836,7 +870,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     hip_check(batch::launch_labels(la, s), "label kernel");
                 }
                 BatchFeatureArgs fa{c->d_box,    c->d_step,    c->d_tile_box,   int(T),
-                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_rho,
+                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_face,
+                                    c->d_tile_rho,
                                     c->d_feat,   nullptr,      nullptr,         0.f,
                                     thr,         nullptr,      0};
                 hip_check(batch::launch_features(fa, s), "feature kernel");
@@ -946,7 +981,7 @@ int m1_batch_host_ms(m1_batch* c, double* setup_ms)
 }
 
 int m1_batch_set_skip_model(m1_batch* c, const m1_mlp* model, float decision, int32_t horizon,
-                            int32_t judge_every, float judge_tol)
+                            int32_t infer_every, int32_t judge_every, float judge_tol)
 {
     return guarded(c, [&] {
         if (model) {
@@ -954,6 +989,8 @@ int m1_batch_set_skip_model(m1_batch* c, const m1_mlp* model, float decision, in
                 throw ValueError{"decision must lie in (0, 1)"};
             if (horizon < 1 || horizon > 255)
                 throw ValueError{"horizon must lie in [1, 255]"};
+            if (infer_every < 1 || infer_every > horizon)
+                throw ValueError{"infer_every must lie in [1, horizon]"};
             if (judge_every < 0)
                 throw ValueError{"judge_every must be >= 0 (0: no judge)"};
             if (!(judge_tol > 0.f))
@@ -966,6 +1003,7 @@ int m1_batch_set_skip_model(m1_batch* c, const m1_mlp* model, float decision, in
         c->model = model;
         c->decision = decision;
         c->horizon = horizon;
+        c->infer_every = infer_every;
         c->judge_every = judge_every;
         c->judge_tol = judge_tol;
         c->judged = c->rejections = 0;

@@ -131,6 +131,7 @@ struct StageArgs32 {
     // evaluate; STAGE2 writes the tile's max phi to tile_max[t].
     const uint8_t* compute;
     float* tile_max;
+    float* tile_face;       // FACE_SLOTS per tile (nullable; needs tile_max)
     int f64_closure;        // face projection, closure, eigenvalues, HLL in double
 };
 
@@ -144,10 +145,36 @@ constexpr int F32_TX = 16, F32_TY = M1_TILE_TY, F32_TZ = 256 / M1_TILE_TY;
 // phi above the threshold are evaluated.
 constexpr int TILE_REACH = 9;
 constexpr int tile_radius(int t) { return (TILE_REACH + t - 1) / t; }
+// Face slabs of a tile, written by STAGE2 when asked: for axis a (x, y, z)
+// and side s (0: low, 1: high), the largest phi within TILE_REACH cells of
+// that face (slot 2a + s) and within TILE_GUARD cells (slot 6 + 2a + s).
+constexpr int TILE_GUARD = 4;
+constexpr int FACE_SLOTS = 12;
+
+// Upper bound of phi over the cells of tile u lying within TILE_REACH
+// (base 0) or TILE_GUARD (base 6) cells of tile t, where u sits at tile
+// offset e = (ex, ey, ez) from t and f holds u's face slabs: the region is the
+// intersection of one slab per nonzero axis (the low side of u when e > 0).
+// Offsets of two or more tiles use the TILE_REACH slab (a superset of the
+// region) and hold nothing within TILE_GUARD (every tile is >= 8 cells wide).
+__device__ inline float facing_bound(const float* f, int ex, int ey, int ez, int base)
+{
+    const int e[3] = {ex, ey, ez};
+    float r = 3.0e38f;
+    for (int k = 0; k < 3; ++k) {
+        if (e[k] == 0)
+            continue;
+        if (base != 0 && (e[k] > 1 || e[k] < -1))
+            return 0.f;
+        r = fminf(r, f[base + 2 * k + (e[k] > 0 ? 0 : 1)]);
+    }
+    return r;
+}
 
 struct TileMaskArgs {
     int ntx, nty, ntz;       // tiles per axis
     const float* tile_max;   // max phi per tile after this step (valid where computed)
+    const float* tile_face;  // face slabs per tile (valid where computed)
     const uint8_t* computed; // tiles evaluated this step
     uint8_t* next;           // tiles to evaluate next step
     float threshold;         // activity threshold on the normalised phi
@@ -227,6 +254,7 @@ struct BatchStageArgs {
     const StepCoef32* coef;  // per box
     float* tile_max;         // per global tile (nullable)
     float* tile_sums;        // per global tile: sum phi, sum |J| (nullable)
+    float* tile_face;        // per global tile: FACE_SLOTS slabs (nullable)
     int wave_mode;
     int f64_closure;         // face projection, closure, eigenvalues, HLL in double
     // stage 2 of boxes flagged in judge (nullable) adds its dose to
@@ -250,6 +278,8 @@ struct BatchCflArgs {
     const StepCoef* coef;      // per box: S0, ed[0], rf, inv_alpha, eig_eps
     unsigned long long* out;   // per box, ordered bits of the maximum
     int wave_mode;
+    int vacuum_aware;          // directional masks from active cells (EXACT speeds only)
+    float act_threshold;       // active: normalised phi above this
 };
 
 struct BatchBoundaryArgs {
@@ -269,6 +299,7 @@ struct BatchMaskArgs {
     const int* tile_box;
     int n_tiles;
     const float* tile_max;
+    const float* tile_face;
     const uint8_t* computed;
     uint8_t* next;
     float threshold;
@@ -276,7 +307,7 @@ struct BatchMaskArgs {
     int all_on;               // no skipping: every tile of an active box
     uint8_t* pred;            // in/out (nullable): steps a tile stays left out
                               // (set by the network, counted down here)
-    uint8_t* ring;            // out (nullable): tiles the dilation adds (not above threshold)
+    uint8_t* ring;            // out (nullable): tiles the network may leave out
     float* wmax;              // in/out (nullable): running max of tile_max
     const uint8_t* only;      // (nullable) boxes not flagged here get no tile
 };
@@ -293,6 +324,7 @@ struct BatchFeatureArgs {
     const uint8_t* computed;
     const float* tile_max;
     const float* tile_sums;
+    const float* tile_face;
     const float* tile_rho;    // mean density per tile, kg/m^3
     float* features;          // out (n_tiles, 11) (nullable)
     // inference (nullable model), on the tiles evaluated in the last step:

@@ -31,6 +31,7 @@ struct TileView {
     const StepCoef32* coef;
     float* tile_max;   // this tile's slot (nullable)
     float* tile_sums;  // this tile's sum phi, sum |J| (nullable)
+    float* tile_face;  // this tile's FACE_SLOTS face slabs (nullable; needs tile_max)
     int wave_mode;
     int tx, ty, tz;    // tile coordinates (x: TX planes, y: TY rows, z: TZ cells)
 };
@@ -477,6 +478,42 @@ __device__ __forceinline__ double cfl_cell(const float* U, long i, long N, doubl
     return ax / (s * g.dx) + ay / (s * g.dy) + az / (s * g.dz);
 }
 
+// cfl_cell with the directional masks of the vacuum-aware CFL
+// (energy.py:610-636): an axis whose mask is off contributes nothing; the
+// z-min inflow speed joins the z axis before its mask is applied.
+__device__ __forceinline__ double cfl_cell_masked(const float* U, long i, long N, double s,
+                                                  const GridDims& g, double eps, bool zmin,
+                                                  double p_in, double rf, bool mx, bool my,
+                                                  bool mz)
+{
+    if (!(mx || my || mz))
+        return 0.0;
+    const double phi = U[i];
+    const double J[3] = {U[i + N], U[i + 2 * N], U[i + 3 * N]};
+    const double jn = sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+    Closure c = closure(phi, J, jn);
+    double lo, hi, ax = 0.0, ay = 0.0, az = 0.0;
+    if (mx) {
+        eigen<0>(phi, c, eps, lo, hi);
+        ax = amax(lo, hi);
+    }
+    if (my) {
+        eigen<1>(phi, c, eps, lo, hi);
+        ay = amax(lo, hi);
+    }
+    if (mz) {
+        eigen<2>(phi, c, eps, lo, hi);
+        az = amax(lo, hi);
+        if (zmin) {
+            const double Jin[3] = {0.0, 0.0, rf * p_in};
+            const Closure ci = closure(p_in, Jin, fabs(Jin[2]));
+            eigen<2>(p_in, ci, eps, lo, hi);
+            az = fmax(az, amax(lo, hi));
+        }
+    }
+    return ax / (s * g.dx) + ay / (s * g.dy) + az / (s * g.dz);
+}
+
 // Inverse CFL step of a cell holding the exact zero state: the isotropic
 // speeds 1/sqrt(3) on every axis (wave_speeds.py:37-50), the same value
 // cfl_cell returns for U = 0.
@@ -589,6 +626,8 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     __shared__ float Fy[4][TY + 1][TZ];
     __shared__ float Fz[4][TY][TZ + 1];
     __shared__ float wmax[3][NT / 64];
+    __shared__ float plane[NT / 64][TX];  // per wave, max phi of each x plane
+    __shared__ float colmax[TY][TZ];      // per (y, z) column, max phi over x
 
     const int tid = threadIdx.x;
     float phimax = 0.f, phisum = 0.f, jsum = 0.f;
@@ -729,6 +768,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
         }
         __syncthreads();
 
+        float xphi = 0.f;  // STAGE2: phi of this cell, for the face slabs
         if (valid) {
             const long i = cidx(x, y, z);
             const int m = a.ids[i];
@@ -799,6 +839,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 for (int c = 0; c < 4; ++c)
                     a.U[i + c * N] = u[c];
                 phimax = fmaxf(phimax, u[0]);
+                xphi = u[0];
                 if (a.tile_sums) {
                     phisum += u[0];
                     jsum += sqrtf(fmaf(u[1], u[1], fmaf(u[2], u[2], u[3] * u[3])));
@@ -813,6 +854,14 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     two_sum_add(a.dose_hi[i], a.dose_lo[i], e * sc.dose_scale[m]);
             }
         }
+        if constexpr (KIND == STAGE2) {
+            if (a.tile_face) {
+                for (int off = 32; off > 0; off >>= 1)
+                    xphi = fmaxf(xphi, __shfl_xor(xphi, off, 64));
+                if ((tid & 63) == 0)
+                    plane[tid >> 6][x - x0] = xphi;
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
             Flo[c] = Fhi[c];
@@ -824,6 +873,8 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     }
     if constexpr (KIND == STAGE2) {
         if (a.tile_max) {
+            if (a.tile_face)
+                colmax[ty][tz] = phimax;
             for (int off = 32; off > 0; off >>= 1) {
                 phimax = fmaxf(phimax, __shfl_xor(phimax, off, 64));
                 phisum += __shfl_xor(phisum, off, 64);
@@ -847,6 +898,32 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     a.tile_sums[0] = ps;
                     a.tile_sums[1] = js;
                 }
+            }
+            if (a.tile_face && tid < FACE_SLOTS) {
+                // slab tid: depth TILE_REACH (slots 0-5) or TILE_GUARD (6-11),
+                // axis (tid % 6) / 2, low (even) or high (odd) side
+                const int depth = tid < 6 ? TILE_REACH : TILE_GUARD;
+                const int axis = (tid % 6) / 2;
+                const bool high = tid & 1;
+                const int ext = axis == 0 ? TX : (axis == 1 ? TY : TZ);
+                const int lo = high ? max(ext - depth, 0) : 0;
+                const int hi = high ? ext : min(depth, ext);
+                float v = 0.f;
+                if (axis == 0) {
+                    const int nxl = x1 - x0;
+                    for (int w = 0; w < NT / 64; ++w)
+                        for (int q = lo; q < min(hi, nxl); ++q)
+                            v = fmaxf(v, plane[w][q]);
+                } else if (axis == 1) {
+                    for (int q = lo; q < hi; ++q)
+                        for (int r = 0; r < TZ; ++r)
+                            v = fmaxf(v, colmax[q][r]);
+                } else {
+                    for (int q = 0; q < TY; ++q)
+                        for (int r = lo; r < hi; ++r)
+                            v = fmaxf(v, colmax[q][r]);
+                }
+                a.tile_face[tid] = v;
             }
         }
     }

@@ -80,6 +80,8 @@ int run_guarded(int device, std::string& err, F&& f)
 struct Num {
     int scheme, wave, scattering, nuclear, interp, precision;
     double max_step, cfl, emin, window, tile_threshold;
+    int vacuum_aware_cfl;
+    int batch_cfl_box_only;
 };
 
 // Validation in the order and wording of config.py:53-89 for the fields the
@@ -90,7 +92,8 @@ inline Num read_numerics(const m1_numerics* n)
         throw ValueError{"numerics must not be NULL"};
     Num r{n->energy_scheme, n->wave_speeds, n->scattering, n->nuclear,
           n->coefficient_interpolation, n->precision, n->max_energy_step, n->cfl,
-          n->energy_min, n->energy_window_sigma, n->tile_threshold};
+          n->energy_min, n->energy_window_sigma, n->tile_threshold, n->vacuum_aware_cfl != 0,
+          n->batch_cfl_box_only != 0};
     if (r.scheme < M1_FORWARD_EULER || r.scheme > M1_IF_SSPRK3)
         throw NotImplemented{"Energy scheme has not been migrated yet."};
     if (r.wave != M1_WAVE_EXACT && r.wave != M1_WAVE_UNIT)

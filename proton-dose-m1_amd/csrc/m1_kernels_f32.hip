@@ -31,9 +31,24 @@ __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(StageArgs32 a
     const long tile = (long(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     if (a.compute && !a.compute[tile])
         return;  // whole tile and its 9-cell neighbourhood are vacuum
-    TileView v{a.g,      a.src,       a.U,   a.V1,  a.out,  a.dose_hi,
-               a.dose_lo, a.edep,     a.ids, a.bnd, a.coef, a.tile_max ? a.tile_max + tile : nullptr,
-               nullptr,  a.wave_mode, int(blockIdx.z), int(blockIdx.y), int(blockIdx.x)};
+    TileView v{a.g,
+               a.src,
+               a.U,
+               a.V1,
+               a.out,
+               a.dose_hi,
+               a.dose_lo,
+               a.edep,
+               a.ids,
+               a.bnd,
+               a.coef,
+               a.tile_max ? a.tile_max + tile : nullptr,
+               nullptr,
+               a.tile_face ? a.tile_face + FACE_SLOTS * tile : nullptr,
+               a.wave_mode,
+               int(blockIdx.z),
+               int(blockIdx.y),
+               int(blockIdx.x)};
     stage_tile<KIND, IFAC, D>(v);
 }
 
@@ -120,8 +135,9 @@ __global__ void scale_kernel(long n, float* U, const uint8_t* ids, const StepCoe
     }
 }
 
-// Next-step tile set: a tile is evaluated when any tile within the dilation
-// radius holds phi above the threshold after this step, or when it is a
+// Next-step tile set: a tile is evaluated when it holds phi above the
+// threshold after this step, when a cell within TILE_REACH cells of it in a
+// neighbouring tile does (bounded by that tile's face slabs), or when it is a
 // z = 0 tile while the inflow spectrum is on.
 __global__ void tile_mask_kernel(TileMaskArgs a)
 {
@@ -130,7 +146,7 @@ __global__ void tile_mask_kernel(TileMaskArgs a)
     if (t >= n)
         return;
     const int tz = t % a.ntz, ty = (t / a.ntz) % a.nty, tx = t / (a.ntz * a.nty);
-    bool on = a.inflow_on && tz == 0;
+    bool on = (a.inflow_on && tz == 0) || (a.computed[t] && a.tile_max[t] > a.threshold);
     for (int dx = -a.rx; dx <= a.rx && !on; ++dx) {
         const int xx = tx + dx;
         if (xx < 0 || xx >= a.ntx)
@@ -141,10 +157,11 @@ __global__ void tile_mask_kernel(TileMaskArgs a)
                 continue;
             for (int dz = -a.rz; dz <= a.rz && !on; ++dz) {
                 const int zz = tz + dz;
-                if (zz < 0 || zz >= a.ntz)
+                if (zz < 0 || zz >= a.ntz || (dx == 0 && dy == 0 && dz == 0))
                     continue;
                 const int u = (xx * a.nty + yy) * a.ntz + zz;
-                on = a.computed[u] && a.tile_max[u] > a.threshold;
+                on = a.computed[u] &&
+                     facing_bound(a.tile_face + FACE_SLOTS * long(u), dx, dy, dz, 0) > a.threshold;
             }
         }
     }

@@ -68,6 +68,7 @@ struct m1_ctx {
     // tile skipping (FP32 bulk)
     uint8_t* d_compute[2] = {nullptr, nullptr};
     float* d_tile_max = nullptr;
+    float* d_tile_face = nullptr;
     int* d_nactive = nullptr;
     int ntx = 0, nty = 0, ntz = 0;
     int cur = 0;
@@ -96,13 +97,14 @@ void m1_ctx::free_problem()
                     (void*)bnd, (void*)spatial, (void*)ids, (void*)U32, (void*)V1_32,
                     (void*)V2_32, (void*)work32, (void*)dose_hi, (void*)dose_lo, (void*)edep32,
                     (void*)bnd32, (void*)d_compute[0], (void*)d_compute[1], (void*)d_tile_max,
+                    (void*)d_tile_face,
                     (void*)d_nactive})
         if (p)
             (void)hipFree(p);
     U = V1 = V2 = work = dose = edep = bnd = spatial = nullptr;
     U32 = V1_32 = V2_32 = work32 = dose_hi = dose_lo = edep32 = bnd32 = nullptr;
     d_compute[0] = d_compute[1] = nullptr;
-    d_tile_max = nullptr;
+    d_tile_max = d_tile_face = nullptr;
     d_nactive = nullptr;
     ids = nullptr;
     have_problem = false;
@@ -348,6 +350,7 @@ void ensure_f32(m1_ctx* c)
     dev_alloc(&c->d_compute[0], nt, "hipMalloc tile set");
     dev_alloc(&c->d_compute[1], nt, "hipMalloc tile set");
     dev_alloc(&c->d_tile_max, nt, "hipMalloc tile max");
+    dev_alloc(&c->d_tile_face, nt * m1::FACE_SLOTS, "hipMalloc tile face slabs");
     dev_alloc(&c->d_nactive, 1, "hipMalloc tile count");
     if (!c->d_step32) {
         dev_alloc(&c->d_step32, 1, "hipMalloc coefficients (f32)");
@@ -400,6 +403,7 @@ void rk3_f32(m1_ctx* c, const Num& n, bool ifac, bool dose, bool edep)
     a.V1 = c->V1_32;
     a.compute = step_tiles(c, n);
     a.tile_max = n.tile_threshold > 0.0 ? c->d_tile_max : nullptr;
+    a.tile_face = n.tile_threshold > 0.0 ? c->d_tile_face : nullptr;
     {
         a.src = c->U32;
         a.out = c->V1_32;
@@ -539,7 +543,7 @@ void update_tiles(m1_ctx* c, const Num& n, double energy_next)
     const double d = (e - c->beam_energy) / c->energy_spread;
     const int inflow_on = std::exp(-0.5 * d * d) > n.tile_threshold ? 1 : 0;
     hip_check(hipMemsetAsync(c->d_nactive, 0, sizeof(int), c->stream), "tile count reset");
-    m1::TileMaskArgs ma{c->ntx, c->nty, c->ntz, c->d_tile_max, c->d_compute[c->cur],
+    m1::TileMaskArgs ma{c->ntx, c->nty, c->ntz, c->d_tile_max, c->d_tile_face, c->d_compute[c->cur],
                         c->d_compute[1 - c->cur], float(n.tile_threshold), inflow_on,
                         m1::tile_radius(m1::F32_TX), m1::tile_radius(m1::F32_TY),
                         m1::tile_radius(m1::F32_TZ), c->d_nactive};

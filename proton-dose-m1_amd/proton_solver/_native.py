@@ -16,7 +16,7 @@ import numpy as np
 from .enums import CoefficientInterpolation, EnergyScheme, Precision, Scattering, WaveSpeeds
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libm1march.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 M1_OK = 0
 M1_ERR_VALUE = -1
@@ -55,7 +55,8 @@ class m1_numerics(C.Structure):
                 ("coefficient_interpolation", C.c_int32), ("precision", C.c_int32),
                 ("max_energy_step", C.c_double), ("cfl", C.c_double),
                 ("energy_min", C.c_double), ("energy_window_sigma", C.c_double),
-                ("tile_threshold", C.c_double)]
+                ("tile_threshold", C.c_double), ("vacuum_aware_cfl", C.c_int32),
+                ("batch_cfl_box_only", C.c_int32)]
 
 
 class m1_beamlet(C.Structure):
@@ -108,7 +109,7 @@ SIGNATURES = [
     ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
     ("m1_batch_set_skip_model", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
-                                          C.c_int32, C.c_float]),
+                                          C.c_int32, C.c_int32, C.c_float]),
     ("m1_batch_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
     ("m1_batch_samples", C.c_int, [C.c_void_p, _fp, _fp, C.c_int64, C.POINTER(C.c_int64)]),
     ("m1_batch_skip_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -185,7 +186,9 @@ def numerics_struct(config, precision_code: int | None = None) -> m1_numerics:
     return m1_numerics(_SCHEME[n.energy_scheme], _WAVE[n.wave_speeds], _SCAT[ph.scattering],
                        int(bool(ph.nuclear)), _INTERP[ph.coefficient_interpolation],
                        int(precision_code), float(n.max_energy_step), float(n.cfl),
-                       float(n.energy_min), float(n.energy_window_sigma), tile)
+                       float(n.energy_min), float(n.energy_window_sigma), tile,
+                       int(bool(getattr(n, "vacuum_aware_cfl", False))),
+                       int(getattr(getattr(config, "device", None), "batch_cfl", "phantom") == "box"))
 
 
 def _table(E, v, slope, keep):
@@ -473,11 +476,13 @@ class Batch:
             out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
         return out
 
-    def set_skip_model(self, model, decision=0.5, horizon=8, judge_every=64, judge_tol=1e-3):
+    def set_skip_model(self, model, decision=0.5, horizon=8, infer_every=4, judge_every=64,
+                       judge_tol=1e-3):
         """Use the tile-skip network `model` (an Mlp, or None to switch it off)."""
         h = None if model is None else model.h
         self._check(self.lib.m1_batch_set_skip_model(self.h, h, float(decision), int(horizon),
-                                                     int(judge_every), float(judge_tol)))
+                                                     int(infer_every), int(judge_every),
+                                                     float(judge_tol)))
         self.model = model  # the library keeps a pointer: keep the model alive
 
     def record(self, enable=True, full_domain=True, horizon=8, capacity=1 << 20):

@@ -136,12 +136,14 @@ def batch_workload(n_beamlets=256, phantom_seed=4, beamlet_seed=40, shape=(200, 
     return phantom, beamlets
 
 
-def batch_workload_config(tile_threshold=1e-10):
+def batch_workload_config(tile_threshold=1e-10, vacuum_aware_cfl=False):
     """Physics and numerics of the batched workloads: Wentzel scattering and
     nuclear removal, WENO-Z, IF-SSPRK3 with exact M1 speeds, max step
-    0.25 MeV, CFL 0.9, energy_min 1 MeV, FP32-bulk with tile skipping.  Its
-    geometry and beam are placeholders: the batch takes the phantom and the
-    beamlets."""
+    0.25 MeV, CFL 0.9 over the whole phantom (as the march of one beamlet on
+    the phantom takes it; vacuum_aware_cfl selects the reference's
+    vacuum-aware rule, energy.py:459-537, instead), energy_min 1 MeV,
+    FP32-bulk with tile skipping.  Its geometry and beam are placeholders: the
+    batch takes the phantom and the beamlets."""
     from .beam import Beam as _Beam
     from .config import DeviceOptions, Numerics, Physics, SolverConfig
     from .enums import EnergyScheme, Scattering, SpatialScheme, WaveSpeeds
@@ -152,7 +154,8 @@ def batch_workload_config(tile_threshold=1e-10):
         physics=Physics(scattering=Scattering.WENTZEL, nuclear=True),
         numerics=Numerics(spatial_scheme=SpatialScheme.WENO_Z,
                           energy_scheme=EnergyScheme.IF_SSPRK3, wave_speeds=WaveSpeeds.EXACT,
-                          max_energy_step=0.25, cfl=0.9, energy_min=1.0),
+                          max_energy_step=0.25, cfl=0.9, energy_min=1.0,
+                          vacuum_aware_cfl=vacuum_aware_cfl),
         device=DeviceOptions(precision=Precision.FP32_BULK, tile_threshold=tile_threshold),
     )
 

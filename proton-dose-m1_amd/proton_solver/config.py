@@ -139,6 +139,10 @@ class DeviceOptions:
     # FP32_BULK only: skip workgroup tiles whose neighbourhood holds phi below
     # this fraction of the peak inflow fluence (0 evaluates every tile).
     tile_threshold: float = 1.0e-10
+    # Batched march: the CFL step of a beamlet bounds, like the march of that
+    # beamlet on the whole phantom, the phantom outside its box too (held at
+    # the zero state there); "box" bounds the box alone.
+    batch_cfl: str = "phantom"
 
     def __post_init__(self):
         if not isinstance(self.precision, Precision):
@@ -147,6 +151,8 @@ class DeviceOptions:
             raise ValueError("device must be a non-negative integer.")
         if not (math.isfinite(self.tile_threshold) and 0.0 <= self.tile_threshold < 1.0):
             raise ValueError("tile_threshold must lie in [0, 1).")
+        if self.batch_cfl not in ("phantom", "box"):
+            raise ValueError('batch_cfl must be "phantom" or "box".')
 
 
 @dataclass

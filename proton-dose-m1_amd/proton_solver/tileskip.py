@@ -1,24 +1,27 @@
 """Learned tile skipping for the batched march (no reference counterpart).
 
-The batched march evaluates a 16x16x16-cell tile when a tile within the
-9-cell dilation radius holds phi above the tile threshold.  Most tiles the
-dilation adds stay empty: the tile-skip network (11 features per tile, three
-hidden layers of 64 ReLU units, bf16 weights, float32 accumulation; the
-device kernels are in csrc/m1_mlp.hip) predicts which of them stay at or
-below the threshold over the next `horizon` steps, and the march leaves those
-out.  Tiles above the threshold and inflow tiles are always evaluated.  A
-periodic judge takes a beamlet's step both ways from the same state; a
-beamlet whose step dose moves by more than `judge_tol` of the step's largest
-increment is sent back to its last accepted checkpoint and marches on with
-the dilation rule alone (include/m1march.h, m1_batch_set_skip_model).
+The batched march evaluates a 16x16x16-cell tile when it holds phi above the
+tile threshold or when a cell within 9 cells of it does (the reach of one
+three-stage step).  Candidate tiles are those the reach rule alone adds while
+nothing above the threshold lies within 4 cells of them; the tile-skip
+network (11 features per tile, three hidden layers of 64 ReLU units, bf16
+weights, float32 accumulation; the device kernels are in csrc/m1_mlp.hip)
+predicts which candidates stay at or below the threshold over the next
+`horizon` steps, and the march leaves those out while they remain
+candidates (checked every step).  Tiles above the threshold and inflow tiles
+are always evaluated.  A periodic judge takes a beamlet's step both ways from
+the same state; a beamlet whose step dose moves by more than `judge_tol` of
+the step's largest increment is sent back to its last accepted checkpoint
+and marches on with the reach rule alone (include/m1march.h,
+m1_batch_set_skip_model).
 
 Training data comes from the march itself: ``record_samples`` runs the
 batched march with every tile of the active boxes evaluated and labels each
-dilation-only tile by whether it stayed at or below the threshold over the
-next `horizon` steps.  ``train`` fits the network on the device.  ``main``
-records on the training workload (its phantom and beamlets are drawn from
-other seeds than the 256-beamlet workload), trains and writes the weights
-that ``load_model`` reads by default.
+candidate tile by whether it stayed at or below the threshold over the next
+`horizon` steps.  ``train`` fits the network on the device.  ``main`` records
+on the training workload (its phantom and beamlets are drawn from other
+seeds than the 256-beamlet workload), trains and writes the weights that
+``load_model`` reads by default.
 """
 
 from __future__ import annotations
@@ -45,7 +48,8 @@ FEATURES = ("log10_max_phi", "log10_mean_phi", "flux_ratio", "density", "energy"
 class SkipOptions:
     decision: float = 0.9      # probability of staying empty needed to leave a tile out
     horizon: int = 8           # steps a tile is left out for (the label's window)
-    judge_every: int = 32      # steps between judged steps of a beamlet
+    infer_every: int = 4       # steps between evaluations of the network
+    judge_every: int = 64      # steps between judged steps of a beamlet
     judge_tol: float = 1e-3    # allowed step-dose change over the step's largest increment
 
 
@@ -118,8 +122,8 @@ def load_model(device=0, path=WEIGHTS_FILE) -> _native.Mlp:
 
 def enable(handle: _native.Batch, model, options: SkipOptions = SkipOptions()):
     """Use `model` (None: switch the network off) in the marches of `handle`."""
-    handle.set_skip_model(model, options.decision, options.horizon, options.judge_every,
-                          options.judge_tol)
+    handle.set_skip_model(model, options.decision, options.horizon, options.infer_every,
+                          options.judge_every, options.judge_tol)
 
 
 def training_workload(n_beamlets=96):

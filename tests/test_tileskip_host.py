@@ -60,4 +60,5 @@ def test_committed_weights_are_well_formed():
 def test_skip_options_defaults():
     o = TS.SkipOptions()
     assert 0.0 < o.decision < 1.0 and 1 <= o.horizon <= 255
+    assert 1 <= o.infer_every <= o.horizon
     assert o.judge_every >= 1 and 0.0 < o.judge_tol <= 1e-3
