@@ -56,9 +56,8 @@ STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
 STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 69}
 # HBM bytes per evaluated cell of the batched stage kernels, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by the same counters on a kernel of
-# known traffic (profiles/stage_kernels_batch_pmc_traffic.md); None until
-# measured for this build.
-PMC_BYTES_PER_CELL_FP32 = None
+# known traffic (profiles/stage_kernels_batch_pmc_traffic.md).
+PMC_BYTES_PER_CELL_FP32 = {"stage0": 63.4, "stage1": 108.0, "stage2": 168.9}
 
 
 def workload_config(precision):

@@ -260,6 +260,20 @@ int m1_batch_tile_stats(m1_batch *batch, int64_t *evaluated, int64_t *full);
 int m1_batch_reset_stats(m1_batch *batch);
 /* Host time of the per-step physics scalars since the last reset, ms. */
 int m1_batch_host_ms(m1_batch *batch, double *setup_ms);
+/* Stage kernels on a persistent grid (the workgroups resident on the
+ * device) that take tiles from eight per-XCD queues, stealing across XCDs
+ * when their own is empty (enable 1), or one workgroup per tile (0, the
+ * default).  Results are identical; only the work distribution changes. */
+int m1_batch_set_persistent(m1_batch *batch, int32_t enable);
+/* Capture a step's launches (step coefficients, boundary traces, the three
+ * persistent stage kernels, the next tile set and the next CFL reduction)
+ * as a HIP graph per tile-set parity and replay it for every step that is
+ * neither judged nor recorded (enable 1), or launch them one by one (0, the
+ * default).  Results are identical. */
+int m1_batch_set_graphs(m1_batch *batch, int32_t enable);
+/* Steps of the last march replayed from a graph; *capture_error (nullable)
+ * is empty, or why a capture failed and the march fell back to launches. */
+int m1_batch_graph_info(m1_batch *batch, int64_t *graph_steps, const char **capture_error);
 
 /* ------------------------------------------------------------------
  * Tile-skip network (no reference counterpart): 11 inputs per workgroup

@@ -40,9 +40,8 @@ __device__ __forceinline__ TileCoord tile_coord(const BoxDesc& d, int b, int t)
 }
 
 template <int KIND, bool IFAC, bool D>
-__global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArgs a)
+__device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
 {
-    const int t = a.list[xcd_slot(int(blockIdx.x), a.n_list)];
     const int b = a.tile_box[t];
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
@@ -73,6 +72,44 @@ __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArg
                tc.ty,
                tc.tz};
     stage_tile<KIND, IFAC, D>(v);
+}
+
+// One workgroup per list entry, or (a.queue set) a persistent grid pulling
+// entries from the per-XCD queues: a workgroup of XCD x (workgroups are
+// dispatched round-robin over the XCDs) drains slice x first, keeping halo
+// re-reads of neighbouring tiles in that XCD's L2, then steals from the
+// others.  Queue counters only grow, so every workgroup ends once all slices
+// are handed out.
+template <int KIND, bool IFAC, bool D>
+__global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArgs a)
+{
+    if (!a.queue) {
+        run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(int(blockIdx.x), a.n_list)]);
+        return;
+    }
+    __shared__ int s_slot;
+    const int n = *a.count;
+    const int q = n / XCD_QUEUES, r = n % XCD_QUEUES;
+    const int home = int(blockIdx.x) % XCD_QUEUES;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int got = -1;
+            for (int k = 0; k < XCD_QUEUES && got < 0; ++k) {
+                const int x = (home + k) % XCD_QUEUES;
+                const int len = q + (x < r ? 1 : 0);
+                const int s = atomicAdd(a.queue + x, 1);
+                if (s < len)
+                    got = x * q + min(x, r) + s;
+            }
+            s_slot = got;
+        }
+        __syncthreads();
+        const int slot = s_slot;
+        if (slot < 0)
+            break;
+        run_tile<KIND, IFAC, D>(a, a.list[slot]);
+        __syncthreads();  // the next tile reuses LDS and s_slot
+    }
 }
 
 // Vacuum-aware CFL of one live tile (energy.py:459-537, 610-636): cell
@@ -311,6 +348,8 @@ __global__ void mask_kernel(BatchMaskArgs a)
 // per 64 tiles; the network's activations live in LDS (m1_mlp.h).
 __global__ void __launch_bounds__(mlp::WG) feature_kernel(BatchFeatureArgs a)
 {
+    if (a.gate && !*a.gate)
+        return;  // uniform: no barrier has run
     __shared__ uint16_t wl[mlp::NPARAM];
     __shared__ float A[mlp::NH][mlp::WG], B[mlp::NH][mlp::WG];
     if (a.w16) {
@@ -473,6 +512,8 @@ constexpr int COMPACT_THREADS = 1024;
 __global__ void __launch_bounds__(COMPACT_THREADS)
     compact_kernel(const uint8_t* flags, int n, int* list, int* count)
 {
+    if (threadIdx.x < 3 * XCD_QUEUES)
+        count[1 + threadIdx.x] = 0;  // the stage queues of this list
     __shared__ int sums[COMPACT_THREADS];
     const int tid = threadIdx.x;
     const int per = (n + COMPACT_THREADS - 1) / COMPACT_THREADS;
@@ -544,8 +585,20 @@ __global__ void box_dose_kernel(const float* hi, const float* lo, long n, double
 
 }  // namespace
 
+int stage_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(&stage_kernel<STAGE2, true, false>), NT, 0) !=
+            hipSuccess ||
+        n < 1)
+        n = 1;
+    return n;
+}
+
 hipError_t launch_stage(int kind, bool ifac, const BatchStageArgs& a, hipStream_t s)
 {
+    // persistent: n_list is the grid (workgroups resident at once)
     if (a.n_list <= 0)
         return hipSuccess;
     const dim3 grid(unsigned(a.n_list)), block(NT);
@@ -651,6 +704,21 @@ hipError_t launch_compare(const BatchCompareArgs& a, int max_cells, hipStream_t 
         return hipSuccess;
     hipLaunchKernelGGL(compare_kernel, dim3(unsigned((max_cells + 255) / 256), unsigned(a.n_boxes)),
                        dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void clear_kernel(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0ull;
+}
+
+hipError_t launch_clear(unsigned long long* p, int n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(clear_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -64,7 +64,13 @@ struct m1_batch {
     double* spatial = nullptr;
     uint8_t* d_flags[2] = {nullptr, nullptr};
     int* d_list = nullptr;
-    int* d_count = nullptr;
+    int* d_count = nullptr;  // LIST_HEADER ints: length, stage queues
+    bool persistent = false; // stage kernels on a fixed grid with XCD queues
+    int persistent_grid = 0; // resident workgroups of a stage kernel
+    bool use_graphs = false; // capture plain steps as HIP graphs
+    std::string graph_error; // why the last capture failed (empty: none)
+    int64_t graph_steps = 0; // steps of the last march run as a graph
+    int *d_gate = nullptr, *h_gate = nullptr;  // network gate of a graph step
     StepCoef* d_coef = nullptr;
     StepCoef* d_cfl = nullptr;
     StepCoef32* d_coef32 = nullptr;
@@ -199,6 +205,10 @@ m1_batch::~m1_batch()
         (void)hipStreamSynchronize(stream);
     free_problem();
     prof.release();
+    if (d_gate)
+        (void)hipFree(d_gate);
+    if (h_gate)
+        (void)hipHostFree(h_gate);
     if (stream)
         (void)hipStreamDestroy(stream);
 }
@@ -266,6 +276,14 @@ int m1_batch_create(m1_batch** out, int device)
     try {
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        int cus = 0;
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
+                  "hipDeviceGetAttribute");
+        const int per_cu = batch::stage_blocks_per_cu();
+        c->persistent_grid = std::max(XCD_QUEUES, (cus * per_cu) / XCD_QUEUES * XCD_QUEUES);
+        dev_alloc(&c->d_gate, 1, "hipMalloc network gate");
+        host_alloc(&c->h_gate, 1, "hipHostMalloc");
+        *c->h_gate = 0;
     } catch (const HipError& e) {
         g_batch_create_error = e.msg;
         delete c;
@@ -431,7 +449,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         dev_alloc(&c->d_flags[0], T, "hipMalloc tile set");
         dev_alloc(&c->d_flags[1], T, "hipMalloc tile set");
         dev_alloc(&c->d_list, T, "hipMalloc tile list");
-        dev_alloc(&c->d_count, 1, "hipMalloc tile count");
+        dev_alloc(&c->d_count, LIST_HEADER, "hipMalloc tile count");
         dev_alloc(&c->d_coef, B, "hipMalloc batch coefficients");
         dev_alloc(&c->d_cfl, B, "hipMalloc batch CFL coefficients");
         dev_alloc(&c->d_coef32, B, "hipMalloc batch coefficients (f32)");
@@ -525,7 +543,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             dev_alloc(&c->d_flags_full, T, "hipMalloc judge tile set");
             dev_alloc(&c->d_list_full, T, "hipMalloc judge tile list");
             hip_check(hipMemset(c->d_list_full, 0, T * sizeof(int)), "judge tile list");
-            dev_alloc(&c->d_count_full, 1, "hipMalloc judge tile count");
+            dev_alloc(&c->d_count_full, LIST_HEADER, "hipMalloc judge tile count");
             dev_alloc(&c->d_jdiff, nb, "hipMalloc judge maxima");
             dev_alloc(&c->d_jfull, nb, "hipMalloc judge maxima");
             host_alloc(&c->h_jdiff, nb, "hipHostMalloc");
@@ -585,27 +603,147 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         const bool skipping = n.tile_threshold > 0.0 && !(c->recording && c->record_full);
         int64_t iter = 0;
 
-        while (n_active > 0) {
-            // 1. CFL maxima of every active box.
-            auto t0 = std::chrono::steady_clock::now();
+        const float thr = float(n.tile_threshold);
+        // Graph steps: the step's launches captured once per tile-set parity
+        // and replayed; they need the persistent stage launch (fixed grid,
+        // list length read on the device).  Judged and recording steps run
+        // as separate launches.
+        bool graphs = c->use_graphs && !c->recording;
+        const bool persistent = c->persistent || graphs;
+
+        // Stage arguments common to every launch of this march.
+        BatchStageArgs a{};
+        a.box = c->d_box;
+        a.tile_box = c->d_tile_box;
+        a.dx = c->dx;
+        a.dy = c->dy;
+        a.dz = c->dz;
+        a.U = c->U;
+        a.V1 = c->V1;
+        a.ids = c->ids;
+        a.bnd = c->bnd;
+        a.coef = c->d_coef32;
+        a.tile_max = skipping || c->recording ? c->d_tile_max : nullptr;
+        a.tile_face = skipping || c->recording ? c->d_tile_face : nullptr;
+        a.tile_sums = use_model || c->recording ? c->d_tile_sums : nullptr;
+        a.wave_mode = n.wave;
+        a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
+        // The three stages over a list; count is its host-known length (the
+        // grid of the one-workgroup-per-tile launch, ignored when persistent).
+        auto stages = [&](const int* list, int* header, int count, float* dhi, float* dlo) {
+            a.list = list;
+            a.count = header;
+            a.queue = nullptr;
+            a.n_list = count;
+            auto persist = [&](int stage) {
+                if (persistent) {
+                    a.n_list = c->persistent_grid;
+                    a.queue = header + 1 + XCD_QUEUES * stage;
+                }
+            };
+            {
+                persist(0);
+                a.src = c->U;
+                a.out = c->V1;
+                a.dose_hi = a.dose_lo = nullptr;
+                a.bnd_stage = 0;
+                Timed t(c->prof, s, K_STAGE0);
+                hip_check(batch::launch_stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
+            }
+            {
+                persist(1);
+                a.src = c->V1;
+                a.out = c->V2;
+                a.bnd_stage = 1;
+                Timed t(c->prof, s, K_STAGE1);
+                hip_check(batch::launch_stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
+            }
+            {
+                persist(2);
+                a.src = c->V2;
+                a.out = nullptr;
+                a.dose_hi = dhi;
+                a.dose_lo = dlo;
+                a.bnd_stage = 2;
+                Timed t(c->prof, s, K_STAGE2);
+                hip_check(batch::launch_stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
+            }
+        };
+        // Step coefficients and boundary traces of the step about to run.
+        auto upload_step = [&]() {
+            hip_check(hipMemcpyAsync(c->d_coef, c->h_coef, size_t(B) * sizeof(StepCoef),
+                                     hipMemcpyHostToDevice, s),
+                      "coefficient upload");
+            hip_check(hipMemcpyAsync(c->d_coef32, c->h_coef32, size_t(B) * sizeof(StepCoef32),
+                                     hipMemcpyHostToDevice, s),
+                      "coefficient upload (f32)");
+            BatchBoundaryArgs ba{c->d_box, c->d_step, B,       c->spatial,
+                                 c->ids,   c->d_coef, c->bnd, ifac ? 1 : 0};
+            Timed t(c->prof, s, K_BOUNDARY);
+            hip_check(batch::launch_boundary(ba, c->max_cols, s), "boundary kernel (batch)");
+        };
+        // Next step's tile set from the state after this step (computed set
+        // flags[k], next set flags[1 - k]); the network runs when gate (device)
+        // or infer (host) says so.
+        auto tail = [&](int k, const int* gate, bool infer) {
+            hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
+                                     hipMemcpyHostToDevice, s),
+                      "box flag upload");
+            if (use_model && (gate || infer)) {
+                // a tile predicted to stay empty is left out for `horizon` steps
+                const mlp::DeviceParams prm = mlp::device_params_of(c->model);
+                BatchFeatureArgs fa{c->d_box,      c->d_step,     c->d_tile_box,   int(T),
+                                    c->d_flags[k], c->d_tile_max, c->d_tile_sums, c->d_tile_face,
+                                    c->d_tile_rho, nullptr,       prm.w,          prm.p,
+                                    c->decision,   thr,           c->d_pred,      c->horizon,
+                                    gate};
+                Timed t(c->prof, s, K_MODEL);
+                hip_check(batch::launch_features(fa, s), "tile-skip network kernel");
+            }
+            {
+                BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
+                                 int(T),        c->d_tile_max,   c->d_tile_face, c->d_flags[k],
+                                 c->d_flags[1 - k], thr,
+                                 tile_radius(F32_TX), tile_radius(F32_TY), tile_radius(F32_TZ),
+                                 skipping ? 0 : 1,
+                                 use_model ? c->d_pred : nullptr,
+                                 c->recording ? c->d_ring : nullptr,
+                                 c->recording ? c->d_wmax : nullptr,
+                                 nullptr};
+                Timed t(c->prof, s, K_TILE_MASK);
+                hip_check(batch::launch_mask(ma, s), "tile mask kernel (batch)");
+                hip_check(batch::launch_compact(c->d_flags[1 - k], int(T), c->d_list, c->d_count,
+                                                s),
+                          "tile compaction kernel");
+            }
+            BatchZeroArgs za{c->d_box,         c->d_tile_box, int(T), c->d_flags[k],
+                             c->d_flags[1 - k], c->U,         c->V1,  c->V2};
+            Timed t(c->prof, s, K_TILE_ZERO);
+            hip_check(batch::launch_zero(za, s), "tile zero kernel (batch)");
+        };
+        // CFL maxima of every active box for the state in U over the tile set
+        // flags[k], read back with the length of the list built from it.
+        auto fill_cfl_host = [&]() {
+            const auto t0 = std::chrono::steady_clock::now();
             for (int b = 0; b < B; ++b)
                 if (c->h_step[b].active) {
                     fill_cfl(c->mats, beam_params(c->bl[size_t(b)]), E[size_t(b)], c->h_cfl[b]);
                     set_norm(c->alpha[size_t(b)], c->h_cfl[b]);
                 }
             c->host_ms += ms_since(t0);
+        };
+        auto cfl_launch = [&](int k) {
             hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, size_t(B) * sizeof(StepCoef),
                                      hipMemcpyHostToDevice, s),
                       "CFL coefficient upload");
-            hip_check(hipMemsetAsync(c->d_red, 0, size_t(B) * sizeof(unsigned long long), s),
-                      "CFL reset");
+            hip_check(batch::launch_clear(c->d_red, B, s), "CFL reset");
             {
-                BatchCflArgs a{c->d_box,  c->d_step,        c->d_tile_box, int(T), c->dx,
-                               c->dy,     c->dz,            c->U,          c->ids, c->spatial,
-                               c->d_flags[cur], c->d_tile_mats, c->d_cfl,  c->d_red, n.wave,
-                               vacuum_aware ? 1 : 0, float(n.tile_threshold)};
+                BatchCflArgs ca{c->d_box,  c->d_step,      c->d_tile_box, int(T), c->dx,
+                                c->dy,     c->dz,          c->U,          c->ids, c->spatial,
+                                c->d_flags[k], c->d_tile_mats, c->d_cfl, c->d_red, n.wave,
+                                vacuum_aware ? 1 : 0, thr};
                 Timed t(c->prof, s, K_CFL);
-                hip_check(batch::launch_cfl(a, s), "CFL kernel (batch)");
+                hip_check(batch::launch_cfl(ca, s), "CFL kernel (batch)");
             }
             hip_check(hipMemcpyAsync(c->h_red, c->d_red, size_t(B) * sizeof(unsigned long long),
                                      hipMemcpyDeviceToHost, s),
@@ -613,12 +751,123 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             hip_check(hipMemcpyAsync(c->h_count, c->d_count, sizeof(int), hipMemcpyDeviceToHost,
                                      s),
                       "tile count read-back");
-            sync(c);
+        };
+        // Everything a step launches after its host work (graph body).
+        auto whole_step = [&](int k, int n_list) {
+            hip_check(hipMemcpyAsync(c->d_gate, c->h_gate, sizeof(int), hipMemcpyHostToDevice, s),
+                      "network gate upload");
+            upload_step();
+            stages(c->d_list, c->d_count, n_list, c->dose_hi, c->dose_lo);
+            tail(k, c->d_gate, false);
+            cfl_launch(1 - k);
+        };
+        hipGraphExec_t graph[2] = {nullptr, nullptr};
+        std::vector<Profiler::Pending> graph_events[2];
+        auto capture = [&](int k, int n_list) -> hipGraphExec_t {
+            const size_t before = c->prof.pending.size();
+            const hipError_t b = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (b != hipSuccess) {
+                c->graph_error = hipGetErrorString(b);
+                (void)hipGetLastError();
+                return nullptr;
+            }
+            std::string why;
+            try {
+                whole_step(k, n_list);
+            } catch (const HipError& e) {
+                why = e.msg;
+            }
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            hipGraphExec_t ex = nullptr;
+            if (why.empty() && e == hipSuccess && g)
+                e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (g)
+                (void)hipGraphDestroy(g);
+            for (size_t i = before; i < c->prof.pending.size(); ++i)
+                graph_events[k].push_back(c->prof.pending[i]);
+            c->prof.pending.resize(before);
+            if (!why.empty() || e != hipSuccess || !ex) {
+                c->graph_error = why.empty() ? std::string(hipGetErrorString(e)) : why;
+                (void)hipGetLastError();
+                c->prof.give_back(graph_events[k]);
+                return nullptr;
+            }
+            return ex;
+        };
+        struct GraphGuard {
+            hipGraphExec_t* g;
+            std::vector<Profiler::Pending>* ev;
+            Profiler& prof;
+            ~GraphGuard()
+            {
+                for (int k = 0; k < 2; ++k) {
+                    if (g[k])
+                        (void)hipGraphExecDestroy(g[k]);
+                    prof.give_back(ev[k]);
+                }
+            }
+        } graph_guard{graph, graph_events, c->prof};
+        c->graph_steps = 0;
+
+        // Energies after a step and next step's box flags; the judge's
+        // accept / roll-back when the step was judged.  Returns roll-backs.
+        auto advance = [&](int n_judge) {
+            n_active = 0;
+            int n_back = 0;
+            for (int b = 0; b < B; ++b) {
+                const size_t k = size_t(b);
+                c->h_step[b].full_set = 0;
+                if (n_judge)
+                    h_back[b] = 0;
+                if (!c->h_step[b].active)
+                    continue;
+                E[k] -= dE[k];
+                steps[k] += 1;
+                since[k] += 1;
+                if (n_judge && h_judge[b]) {
+                    // accepted: the pending checkpoint becomes the last good
+                    // one; rejected: back to the last good checkpoint, on
+                    // every tile for one step, without the network after
+                    float df, fl;
+                    std::memcpy(&df, &c->h_jdiff[b], sizeof df);
+                    std::memcpy(&fl, &c->h_jfull[b], sizeof fl);
+                    c->judged += 1;
+                    if (!(df <= c->judge_tol * fl) && !(fl == 0.f && df == 0.f)) {
+                        c->rejected[k] = 1;
+                        c->rejections += 1;
+                        h_back[b] = uint8_t(acc[k] < 0 ? 3 : 1 + acc[k]);
+                        E[k] = acc[k] < 0 ? E0[k] : ckE[acc[k]][k];
+                        steps[k] = acc[k] < 0 ? 0 : ckS[acc[k]][k];
+                        c->rollback_to[k] = steps[k];
+                        c->h_step[b].full_set = 1;
+                        n_back += 1;
+                    } else {
+                        acc[k] = pend[k];
+                        since[k] = 1;
+                    }
+                }
+                const bool on = E[k] > n.emin;
+                c->h_step[b].active = on ? 1 : 0;
+                c->h_step[b].inflow_on = on ? inflow_on(c->bl[k], n, E[k]) : 0;
+                c->h_step[b].model_on = use_model && !c->rejected[k] ? 1 : 0;
+                c->h_step[b].energy = float(E[k]);
+                n_active += on ? 1 : 0;
+            }
+            return n_back;
+        };
+
+        // CFL of the first step.
+        fill_cfl_host();
+        cfl_launch(cur);
+        sync(c);
+
+        while (n_active > 0) {
             const int n_list = *c->h_count;
             c->prof.tiles_evaluated += n_list;
 
-            // 2. Step scalars of every active beamlet.
-            t0 = std::chrono::steady_clock::now();
+            // 1. Step scalars of every active beamlet.
+            auto t0 = std::chrono::steady_clock::now();
             for (int b = 0; b < B; ++b) {
                 if (!c->h_step[b].active)
                     continue;
@@ -668,66 +917,39 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     n_judge += 1;
                     n_full += tiles_of[k];
                 }
-            hip_check(hipMemcpyAsync(c->d_coef, c->h_coef, size_t(B) * sizeof(StepCoef),
-                                     hipMemcpyHostToDevice, s),
-                      "coefficient upload");
-            hip_check(hipMemcpyAsync(c->d_coef32, c->h_coef32, size_t(B) * sizeof(StepCoef32),
-                                     hipMemcpyHostToDevice, s),
-                      "coefficient upload (f32)");
 
-            // 3. Boundary traces and the three stages over the tile list.
-            {
-                BatchBoundaryArgs ba{c->d_box, c->d_step, B,       c->spatial,
-                                     c->ids,   c->d_coef, c->bnd, ifac ? 1 : 0};
-                Timed t(c->prof, s, K_BOUNDARY);
-                hip_check(batch::launch_boundary(ba, c->max_cols, s), "boundary kernel (batch)");
+            if (graphs && n_judge == 0) {
+                // 2a. The host work of the whole step first, then its launches
+                // as one graph (no judge result to wait for).
+                (void)advance(0);
+                if (n_active == 0) {
+                    upload_step();
+                    stages(c->d_list, c->d_count, n_list, c->dose_hi, c->dose_lo);
+                    break;
+                }
+                iter += 1;
+                *c->h_gate = use_model && iter % c->infer_every == 0 ? 1 : 0;
+                fill_cfl_host();
+                if (!graph[cur] && graphs)
+                    graph[cur] = capture(cur, n_list);
+                if (graph[cur]) {
+                    hip_check(hipGraphLaunch(graph[cur], s), "hipGraphLaunch");
+                    c->graph_steps += 1;
+                } else {
+                    graphs = false;
+                    whole_step(cur, n_list);
+                }
+                sync(c);
+                if (graph[cur] && c->prof.on)
+                    c->prof.collect_graph(graph_events[cur]);
+                cur = 1 - cur;
+                continue;
             }
-            BatchStageArgs a{};
-            a.box = c->d_box;
-            a.tile_box = c->d_tile_box;
-            a.dx = c->dx;
-            a.dy = c->dy;
-            a.dz = c->dz;
-            a.U = c->U;
-            a.V1 = c->V1;
-            a.ids = c->ids;
-            a.bnd = c->bnd;
-            a.coef = c->d_coef32;
-            a.tile_max = skipping || c->recording ? c->d_tile_max : nullptr;
-            a.tile_face = skipping || c->recording ? c->d_tile_face : nullptr;
-            a.tile_sums = use_model || c->recording ? c->d_tile_sums : nullptr;
-            a.wave_mode = n.wave;
-            a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
-            auto stages = [&](const int* list, int count, float* dhi, float* dlo) {
-                a.list = list;
-                a.n_list = count;
-                {
-                    a.src = c->U;
-                    a.out = c->V1;
-                    a.dose_hi = a.dose_lo = nullptr;
-                    a.bnd_stage = 0;
-                    Timed t(c->prof, s, K_STAGE0);
-                    hip_check(batch::launch_stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
-                }
-                {
-                    a.src = c->V1;
-                    a.out = c->V2;
-                    a.bnd_stage = 1;
-                    Timed t(c->prof, s, K_STAGE1);
-                    hip_check(batch::launch_stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
-                }
-                {
-                    a.src = c->V2;
-                    a.out = nullptr;
-                    a.dose_hi = dhi;
-                    a.dose_lo = dlo;
-                    a.bnd_stage = 2;
-                    Timed t(c->prof, s, K_STAGE2);
-                    hip_check(batch::launch_stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
-                }
-            };
+
+            // 2b. Step by separate launches.
+            upload_step();
             if (n_judge == 0) {
-                stages(c->d_list, n_list, c->dose_hi, c->dose_lo);
+                stages(c->d_list, c->d_count, n_list, c->dose_hi, c->dose_lo);
             } else {
                 // The judged beamlets take the step twice from the same state:
                 // on every tile of their box (dose into the full pair), then
@@ -748,7 +970,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                                 c->d_count_full, s),
                           "judge tile list");
                 c->prof.tiles_evaluated += n_full;
-                stages(c->d_list_full, n_full, c->jf_hi, c->jf_lo);
+                stages(c->d_list_full, c->d_count_full, n_full, c->jf_hi, c->jf_lo);
                 slices(SLICE_RESTORE, d_save);
                 BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags_full,
                                  c->d_flags[cur], c->U, c->V1, c->V2};
@@ -756,7 +978,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 a.judge = d_judge;
                 a.judge_hi = c->js_hi;
                 a.judge_lo = c->js_lo;
-                stages(c->d_list, n_list, c->dose_hi, c->dose_lo);
+                stages(c->d_list, c->d_count, n_list, c->dose_hi, c->dose_lo);
                 a.judge = nullptr;
                 slices(SLICE_ADD, d_judge);
                 hip_check(hipMemsetAsync(c->d_jdiff, 0, nb * sizeof(unsigned), s),
@@ -775,48 +997,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 sync(c);
             }
 
-            // 4. Energies and the next step's tile sets.
-            n_active = 0;
-            int n_back = 0;
-            for (int b = 0; b < B; ++b) {
-                const size_t k = size_t(b);
-                c->h_step[b].full_set = 0;
-                if (n_judge)
-                    h_back[b] = 0;
-                if (!c->h_step[b].active)
-                    continue;
-                E[k] -= dE[k];
-                steps[k] += 1;
-                since[k] += 1;
-                if (n_judge && h_judge[b]) {
-                    // accepted: the pending checkpoint becomes the last good
-                    // one; rejected: back to the last good checkpoint, on
-                    // every tile for one step, without the network after
-                    float df, fl;
-                    std::memcpy(&df, &c->h_jdiff[b], sizeof df);
-                    std::memcpy(&fl, &c->h_jfull[b], sizeof fl);
-                    c->judged += 1;
-                    if (!(df <= c->judge_tol * fl) && !(fl == 0.f && df == 0.f)) {
-                        c->rejected[k] = 1;
-                        c->rejections += 1;
-                        h_back[b] = uint8_t(acc[k] < 0 ? 3 : 1 + acc[k]);
-                        E[k] = acc[k] < 0 ? E0[k] : ckE[acc[k]][k];
-                        steps[k] = acc[k] < 0 ? 0 : ckS[acc[k]][k];
-                        c->rollback_to[k] = steps[k];
-                        c->h_step[b].full_set = 1;
-                        n_back += 1;
-                    } else {
-                        acc[k] = pend[k];
-                        since[k] = 1;
-                    }
-                }
-                const bool on = E[k] > n.emin;
-                c->h_step[b].active = on ? 1 : 0;
-                c->h_step[b].inflow_on = on ? inflow_on(c->bl[k], n, E[k]) : 0;
-                c->h_step[b].model_on = use_model && !c->rejected[k] ? 1 : 0;
-                c->h_step[b].energy = float(E[k]);
-                n_active += on ? 1 : 0;
-            }
+            // 3. Energies and the next step's tile sets.
+            const int n_back = advance(n_judge);
             if (n_back) {
                 hip_check(hipMemcpyAsync(c->d_sel + 2 * nb, h_back, nb, hipMemcpyHostToDevice, s),
                           "roll-back flag upload");
@@ -825,43 +1007,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             if (n_active == 0)
                 break;
             iter += 1;
-            hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
-                                     hipMemcpyHostToDevice, s),
-                      "box flag upload");
-            const float thr = float(n.tile_threshold);
-            if (use_model && iter % c->infer_every == 0) {
-                // a tile predicted to stay empty is left out for `horizon` steps
-                const mlp::DeviceParams prm = mlp::device_params_of(c->model);
-                BatchFeatureArgs fa{c->d_box,    c->d_step,    c->d_tile_box,   int(T),
-                                    c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_face,
-                                    c->d_tile_rho,
-                                    nullptr,     prm.w,        prm.p,           c->decision,
-                                    thr,         c->d_pred,    c->horizon};
-                Timed t(c->prof, s, K_MODEL);
-                hip_check(batch::launch_features(fa, s), "tile-skip network kernel");
-            }
-            {
-                BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
-                                 int(T),        c->d_tile_max,   c->d_tile_face, c->d_flags[cur],
-                                 c->d_flags[1 - cur], thr,
-                                 tile_radius(F32_TX), tile_radius(F32_TY), tile_radius(F32_TZ),
-                                 skipping ? 0 : 1,
-                                 use_model ? c->d_pred : nullptr,
-                                 c->recording ? c->d_ring : nullptr,
-                                 c->recording ? c->d_wmax : nullptr,
-                                 nullptr};
-                Timed t(c->prof, s, K_TILE_MASK);
-                hip_check(batch::launch_mask(ma, s), "tile mask kernel (batch)");
-                hip_check(batch::launch_compact(c->d_flags[1 - cur], int(T), c->d_list,
-                                                c->d_count, s),
-                          "tile compaction kernel");
-            }
-            {
-                BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags[cur],
-                                 c->d_flags[1 - cur], c->U, c->V1, c->V2};
-                Timed t(c->prof, s, K_TILE_ZERO);
-                hip_check(batch::launch_zero(za, s), "tile zero kernel (batch)");
-            }
+            tail(cur, nullptr, use_model && iter % c->infer_every == 0);
             if (c->recording && iter % c->record_horizon == 0) {
                 // label the ring saved `horizon` steps ago, then save this step's
                 if (iter >= c->record_horizon) {
@@ -873,12 +1019,16 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                     c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_face,
                                     c->d_tile_rho,
                                     c->d_feat,   nullptr,      nullptr,         0.f,
-                                    thr,         nullptr,      0};
+                                    thr,         nullptr,      0,               nullptr};
                 hip_check(batch::launch_features(fa, s), "feature kernel");
                 hip_check(hipMemcpyAsync(c->d_cand, c->d_ring, T, hipMemcpyDeviceToDevice, s),
                           "candidate save");
                 hip_check(hipMemsetAsync(c->d_wmax, 0, T * sizeof(float), s), "window reset");
             }
+            // 4. CFL of the next step.
+            fill_cfl_host();
+            cfl_launch(1 - cur);
+            sync(c);
             cur = 1 - cur;
         }
 
@@ -977,6 +1127,33 @@ int m1_batch_host_ms(m1_batch* c, double* setup_ms)
     return guarded(c, [&] {
         if (setup_ms)
             *setup_ms = c->host_ms;
+    });
+}
+
+int m1_batch_set_graphs(m1_batch* c, int32_t enable)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->use_graphs = enable != 0;
+        c->graph_error.clear();
+    });
+}
+
+int m1_batch_graph_info(m1_batch* c, int64_t* graph_steps, const char** capture_error)
+{
+    return guarded(c, [&] {
+        if (graph_steps)
+            *graph_steps = c->graph_steps;
+        if (capture_error)
+            *capture_error = c->graph_error.c_str();
+    });
+}
+
+int m1_batch_set_persistent(m1_batch* c, int32_t enable)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->persistent = enable != 0;
     });
 }
 

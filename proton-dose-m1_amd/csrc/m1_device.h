@@ -262,7 +262,18 @@ struct BatchStageArgs {
     const uint8_t* judge;
     float* judge_hi;
     float* judge_lo;
+    // Persistent launch (queue not null): a fixed grid whose workgroups take
+    // list entries from per-XCD queues (count[0] entries; the list split into
+    // XCD_QUEUES contiguous slices, queue[x] counting the entries of slice x
+    // handed out), stealing from the other slices once their own is empty.
+    const int* count;
+    int* queue;
 };
+
+// Per list: its length followed by one set of XCD queue counters per stage
+// (the compaction that writes the list clears them).
+constexpr int XCD_QUEUES = 8;
+constexpr int LIST_HEADER = 1 + 3 * XCD_QUEUES;
 
 struct BatchCflArgs {
     const BoxDesc* box;
@@ -336,6 +347,7 @@ struct BatchFeatureArgs {
     float threshold;
     uint8_t* pred;
     int horizon;
+    const int* gate;          // (nullable) the kernel does nothing while *gate == 0
 };
 
 // Recording: append (features, label) of the ring tiles saved `horizon`
@@ -417,6 +429,10 @@ hipError_t launch_features(const BatchFeatureArgs& a, hipStream_t s);
 hipError_t launch_labels(const BatchLabelArgs& a, hipStream_t s);
 hipError_t launch_compare(const BatchCompareArgs& a, int max_cells, hipStream_t s);
 hipError_t launch_slices(const BatchSliceArgs& a, int max_cells, hipStream_t s);
+// p[0 .. n) = 0 (a kernel, so that graphs hold no memset node).
+hipError_t launch_clear(unsigned long long* p, int n, hipStream_t s);
+// Workgroups of the stage kernels resident per CU (occupancy).
+int stage_blocks_per_cu();
 }  // namespace batch
 
 namespace f32 {

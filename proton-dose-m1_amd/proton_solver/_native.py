@@ -108,6 +108,10 @@ SIGNATURES = [
                                       C.POINTER(C.c_int64)]),
     ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
+    ("m1_batch_set_persistent", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_set_graphs", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_graph_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_char_p)]),
     ("m1_batch_set_skip_model", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
                                           C.c_int32, C.c_int32, C.c_float]),
     ("m1_batch_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
@@ -475,6 +479,23 @@ class Batch:
             self._check(self.lib.m1_batch_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
             out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
         return out
+
+    def set_persistent(self, enable: bool):
+        """Persistent stage kernels with per-XCD tile queues (True) or one
+        workgroup per tile (False)."""
+        self._check(self.lib.m1_batch_set_persistent(self.h, int(bool(enable))))
+
+    def set_graphs(self, enable: bool):
+        """Replay plain steps from captured HIP graphs (True) or launch them
+        one by one (False)."""
+        self._check(self.lib.m1_batch_set_graphs(self.h, int(bool(enable))))
+
+    def graph_info(self):
+        """(steps of the last march replayed from a graph, capture error or '')."""
+        n = C.c_int64()
+        err = C.c_char_p()
+        self._check(self.lib.m1_batch_graph_info(self.h, C.byref(n), C.byref(err)))
+        return n.value, (err.value or b"").decode()
 
     def set_skip_model(self, model, decision=0.5, horizon=8, infer_every=4, judge_every=64,
                        judge_tol=1e-3):

@@ -200,3 +200,19 @@ def test_batch_argument_checks():
     with pytest.raises(ValueError, match="outside the phantom"):
         h.set_problem(ph.ids, ph.spacing, ph.materials, bad, profiles,
                       [bl.beam() for bl in beamlets])
+
+
+def test_persistent_launch_is_bitwise_the_per_tile_launch():
+    """Stage kernels on a persistent grid with per-XCD tile queues compute the
+    same tiles with the same arithmetic as one workgroup per tile."""
+    ph, beamlets, boxes = small_workload()
+    cfg = _Cfg()
+    num = _native.numerics_struct(cfg)
+    h, _ = prepare_batch(ph, beamlets, cfg, boxes)
+    d0, s0 = h.march(num)
+    h.set_persistent(True)
+    d1, s1 = h.march(num)
+    h.set_persistent(False)
+    say("persistent launch: steps", s1.tolist(), "bitwise", bool(np.array_equal(d0, d1)))
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1)
