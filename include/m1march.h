@@ -269,7 +269,8 @@ int m1_batch_set_persistent(m1_batch *batch, int32_t enable);
  * persistent stage kernels, the next tile set and the next CFL reduction)
  * as a HIP graph per tile-set parity and replay it for every step that is
  * neither judged nor recorded (enable 1), or launch them one by one (0, the
- * default).  Results are identical. */
+ * default).  Results are identical.  With kernel timing on
+ * (m1_batch_set_profiling) every step is launched one by one. */
 int m1_batch_set_graphs(m1_batch *batch, int32_t enable);
 /* Steps of the last march replayed from a graph; *capture_error (nullable)
  * is empty, or why a capture failed and the march fell back to launches. */

@@ -607,8 +607,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // Graph steps: the step's launches captured once per tile-set parity
         // and replayed; they need the persistent stage launch (fixed grid,
         // list length read on the device).  Judged and recording steps run
-        // as separate launches.
-        bool graphs = c->use_graphs && !c->recording;
+        // as separate launches, and so does every step while kernels are
+        // timed (the per-launch event pairs stay outside graphs).
+        bool graphs = c->use_graphs && !c->recording && !c->prof.on;
         const bool persistent = c->persistent || graphs;
 
         // Stage arguments common to every launch of this march.
@@ -762,9 +763,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             cfl_launch(1 - k);
         };
         hipGraphExec_t graph[2] = {nullptr, nullptr};
-        std::vector<Profiler::Pending> graph_events[2];
         auto capture = [&](int k, int n_list) -> hipGraphExec_t {
-            const size_t before = c->prof.pending.size();
             const hipError_t b = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             if (b != hipSuccess) {
                 c->graph_error = hipGetErrorString(b);
@@ -784,30 +783,22 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
             if (g)
                 (void)hipGraphDestroy(g);
-            for (size_t i = before; i < c->prof.pending.size(); ++i)
-                graph_events[k].push_back(c->prof.pending[i]);
-            c->prof.pending.resize(before);
             if (!why.empty() || e != hipSuccess || !ex) {
                 c->graph_error = why.empty() ? std::string(hipGetErrorString(e)) : why;
                 (void)hipGetLastError();
-                c->prof.give_back(graph_events[k]);
                 return nullptr;
             }
             return ex;
         };
         struct GraphGuard {
             hipGraphExec_t* g;
-            std::vector<Profiler::Pending>* ev;
-            Profiler& prof;
             ~GraphGuard()
             {
-                for (int k = 0; k < 2; ++k) {
+                for (int k = 0; k < 2; ++k)
                     if (g[k])
                         (void)hipGraphExecDestroy(g[k]);
-                    prof.give_back(ev[k]);
-                }
             }
-        } graph_guard{graph, graph_events, c->prof};
+        } graph_guard{graph};
         c->graph_steps = 0;
 
         // Energies after a step and next step's box flags; the judge's
@@ -940,8 +931,6 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     whole_step(cur, n_list);
                 }
                 sync(c);
-                if (graph[cur] && c->prof.on)
-                    c->prof.collect_graph(graph_events[cur]);
                 cur = 1 - cur;
                 continue;
             }

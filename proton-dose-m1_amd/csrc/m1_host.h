@@ -358,27 +358,6 @@ struct Profiler {
         }
         pending.clear();
     }
-    // Fold the event pairs of a replayed graph into the statistics (after a
-    // stream sync); the pairs stay with the graph.
-    void collect_graph(const std::vector<Pending>& v)
-    {
-        for (const auto& p : v) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-                launches[p.kernel] += 1;
-                total_ms[p.kernel] += ms;
-            }
-        }
-    }
-    // Return a graph's event pairs to the pool.
-    void give_back(std::vector<Pending>& v)
-    {
-        for (auto& p : v) {
-            pool.push_back(p.a);
-            pool.push_back(p.b);
-        }
-        v.clear();
-    }
     void reset()
     {
         tiles_evaluated = tiles_full = 0;

@@ -216,3 +216,28 @@ def test_persistent_launch_is_bitwise_the_per_tile_launch():
     say("persistent launch: steps", s1.tolist(), "bitwise", bool(np.array_equal(d0, d1)))
     assert np.array_equal(s0, s1)
     assert np.array_equal(d0, d1)
+
+
+def test_graph_steps_are_bitwise_the_launched_steps():
+    """Plain steps replayed from captured HIP graphs (persistent stage
+    kernels, the next tile set and CFL reduction inside the graph) give the
+    march of separate launches bit for bit."""
+    ph, beamlets, boxes = small_workload()
+    cfg = _Cfg()
+    num = _native.numerics_struct(cfg)
+    h, _ = prepare_batch(ph, beamlets, cfg, boxes)
+    d0, s0 = h.march(num)
+    h.set_graphs(True)
+    d1, s1 = h.march(num)
+    n_graph, err = h.graph_info()
+    # with kernel timing on, every step is launched one by one
+    h.set_profiling(True)
+    d2, s2 = h.march(num)
+    n_timed, _ = h.graph_info()
+    h.set_profiling(False)
+    h.set_graphs(False)
+    say("graph steps", n_graph, "of", int(s1.max()), "capture error", repr(err),
+        "graph steps while timed", n_timed)
+    assert err == "" and n_graph > 0.9 * int(s1.max()) and n_timed == 0
+    assert np.array_equal(s0, s1) and np.array_equal(s0, s2)
+    assert np.array_equal(d0, d1) and np.array_equal(d0, d2)

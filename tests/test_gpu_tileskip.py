@@ -131,6 +131,25 @@ def test_never_skipping_network_is_bitwise_the_plain_march(target):
     assert np.array_equal(dose, dose0)
 
 
+def test_graph_steps_with_the_network_are_bitwise_the_launched_steps(target):
+    """Network on: plain steps run as graphs (the network gated on the
+    device), judged steps as separate launches; same march bit for bit."""
+    handle, num, dose0, steps0, tiles0, per0 = target
+    model = TS.load_model(0) if TS.WEIGHTS_FILE.exists() else _constant_model(30.0)
+    opts = TS.SkipOptions(judge_every=16)
+    d_launch, s_launch, *_ = _march(handle, num, model, opts)
+    handle.set_graphs(True)
+    try:
+        d_graph, s_graph, tiles, judged, rejections, back, _ = _march(handle, num, model, opts)
+        n_graph, err = handle.graph_info()
+    finally:
+        handle.set_graphs(False)
+    say("network with graphs: graph steps", n_graph, "judged", judged, "error", repr(err))
+    assert err == "" and n_graph > 0 and judged > 0
+    assert np.array_equal(s_launch, s_graph)
+    assert np.array_equal(d_launch, d_graph)
+
+
 def test_judge_rolls_back_an_always_skipping_network(target):
     handle, num, dose0, steps0, tiles0, per0 = target
     dose, steps, tiles, judged, rejections, back, per = _march(
