@@ -183,10 +183,27 @@ def run_batch(args):
 
     cfg = batch_config(args.tile_threshold)
     phantom, beamlets = batch_workload(n_beamlets=args.beamlets)
+    # Tile shape and launch from the autotuner (a sample of the beamlets,
+    # outside the timed region).
+    tuned = None
+    if args.autotune:
+        from proton_solver.autotune import autotune
+
+        t0 = time.perf_counter()
+        choice = autotune(phantom, beamlets, cfg, sample=args.autotune_sample,
+                          modes=("per_tile", "persistent"),
+                          log=lambda row: print(json.dumps({"autotune": row}), file=sys.stderr,
+                                                flush=True))
+        tuned = {"shape": list(choice.shape), "mode": choice.mode,
+                 "sample_beamlets": args.autotune_sample,
+                 "seconds": round(time.perf_counter() - t0, 2), "table": choice.table}
     t0 = time.perf_counter()
     boxes = beamlet_boxes(phantom, beamlets, cfg.numerics.energy_min,
                           cfg.numerics.energy_window_sigma)
-    handle, boxes = prepare_batch(phantom, beamlets, cfg, boxes)
+    handle, boxes = prepare_batch(phantom, beamlets, cfg, boxes,
+                                  tile_shape=choice.shape_index if tuned else 0)
+    if tuned:
+        choice.apply(handle)
     setup_s = time.perf_counter() - t0
     num = _native.numerics_struct(cfg)
     from proton_solver import tileskip
@@ -243,7 +260,7 @@ def run_batch(args):
             })
         print(json.dumps({"tile_skip_network": net}), file=sys.stderr, flush=True)
     box_cells = np.array([b[2] * b[3] * b[4] for b in boxes], dtype=np.float64)
-    tx, ty, tz = _native.tile_shape()
+    tx, ty, tz = handle.tile_shape
     tile_cap = np.array([-(-b[2] // tx) * -(-b[3] // ty) * -(-b[4] // tz) * (tx * ty * tz)
                          for b in boxes], dtype=np.float64)
     fill = box_cells.sum() / tile_cap.sum()
@@ -336,6 +353,8 @@ def run_batch(args):
                         "FP32 bulk with tile skipping" +
                         (" and the tile-skip network" if use_net else ""),
             "n_beamlets": B,
+            "tile_shape": list(handle.tile_shape),
+            "launch": tuned["mode"] if tuned else "per_tile",
             "box_cells_total": int(box_cells.sum()),
             "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
                                          "max": int(steps.max())},
@@ -346,6 +365,7 @@ def run_batch(args):
         "full_domain_equivalent_cell_steps_per_s": float(n_phantom * steps.sum()) * K / elapsed,
         "tiles_evaluated_fraction": eval_frac,
         "tile_skip_network": net,
+        "autotune": tuned,
         "host_step_setup_ms_per_batch": host_ms / K,
         "setup_s": setup_s,
         "kernel_ms_per_step": {k: round(v[1] / K, 3) for k, v in stats.items() if v[0]},
@@ -375,6 +395,9 @@ def main():
                     help="tile-skip network in the batched march (auto: when its weights exist)")
     ap.add_argument("--no-network-reference", action="store_true",
                     help="skip the untimed march without the network")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="default tile shape and launch instead of the autotuner's choice")
+    ap.add_argument("--autotune-sample", type=int, default=12)
     ap.add_argument("--profile-calibration", action="store_true",
                     help="run a kernel of known HBM traffic after the timed march")
     args = ap.parse_args()

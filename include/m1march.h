@@ -265,6 +265,13 @@ int m1_batch_host_ms(m1_batch *batch, double *setup_ms);
  * when their own is empty (enable 1), or one workgroup per tile (0, the
  * default).  Results are identical; only the work distribution changes. */
 int m1_batch_set_persistent(m1_batch *batch, int32_t enable);
+/* Tile shapes the batched kernels are compiled for (cells per tile along x,
+ * y, z; index 0 is the default, 16 x 16 x 16).  m1_batch_set_tile_shape
+ * selects one for the batch and must precede m1_batch_set_problem.  No
+ * reference counterpart; proton_solver/autotune.py times them. */
+int32_t m1_batch_tile_shape_count(void);
+int m1_batch_tile_shape(int32_t index, int32_t *tx, int32_t *ty, int32_t *tz);
+int m1_batch_set_tile_shape(m1_batch *batch, int32_t index);
 /* Capture a step's launches (step coefficients, boundary traces, the three
  * persistent stage kernels, the next tile set and the next CFL reduction)
  * as a HIP graph per tile-set parity and replay it for every step that is

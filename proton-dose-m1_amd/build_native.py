@@ -12,6 +12,7 @@ proton_solver/_lib/.
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 from pathlib import Path
@@ -36,11 +37,20 @@ COMMON = [
 ]
 
 
+def batch_shapes():
+    """Tile shapes of the batched kernels, read from M1_BATCH_SHAPES in
+    csrc/m1_device.h (the one list the library is built from)."""
+    text = (CSRC / "m1_device.h").read_text()
+    m = re.search(r"#define M1_BATCH_SHAPES\(X\)(.*)", text)
+    shapes = [tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+), (\d+)\)", m.group(1))]
+    if not shapes:
+        raise RuntimeError("no M1_BATCH_SHAPES in m1_device.h")
+    return shapes
+
+
 def _units():
     waves = [f"-DM1_STAGE_WAVES={int(os.environ['M1_STAGE_WAVES'])}"] \
         if os.environ.get("M1_STAGE_WAVES") else []
-    if os.environ.get("M1_TILE_TY"):
-        waves.append(f"-DM1_TILE_TY={int(os.environ['M1_TILE_TY'])}")
     return [
         (CSRC / "m1_kernels.hip", BUILD / "m1_kernels_exact.o",
          ["-DM1_EXACT=1", "-ffp-contract=off"]),
@@ -50,8 +60,10 @@ def _units():
         # four moments of a cell round alike (m1_f32_common.h)
         (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=off", *waves]),
         (CSRC / "m1_march.hip", BUILD / "m1_march.o", ["-ffp-contract=off", *waves]),
-        (CSRC / "m1_batch_kernels.hip", BUILD / "m1_batch_kernels.o",
-         ["-ffp-contract=off", *waves]),
+        *[(CSRC / "m1_batch_kernels.hip", BUILD / f"m1_batch_kernels_{x}x{y}x{z}.o",
+           ["-ffp-contract=off", *waves, f"-DM1_TILE_TX={x}", f"-DM1_TILE_TY={y}",
+            f"-DM1_BATCH_NS=batch_{x}x{y}x{z}", f"-DM1_BATCH_TABLE=batch_kernels_{x}x{y}x{z}"])
+          for x, y, z in batch_shapes()],
         (CSRC / "m1_batch_march.hip", BUILD / "m1_batch_march.o", ["-ffp-contract=off", *waves]),
         (CSRC / "m1_mlp.hip", BUILD / "m1_mlp.o", ["-ffp-contract=off"]),
     ]

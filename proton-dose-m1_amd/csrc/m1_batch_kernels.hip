@@ -11,8 +11,16 @@
 #include "m1_f32_common.h"
 #include "m1_mlp.h"
 
+// Compiled once per tile shape: M1_TILE_TX / M1_TILE_TY set the shape,
+// M1_BATCH_NS the namespace of this copy's launchers and M1_BATCH_TABLE the
+// function returning their table (m1_device.h, M1_BATCH_SHAPES).
+#ifndef M1_BATCH_NS
+#define M1_BATCH_NS batch_16x16x16
+#define M1_BATCH_TABLE batch_kernels_16x16x16
+#endif
+
 namespace m1 {
-namespace batch {
+namespace M1_BATCH_NS {
 namespace {
 
 using namespace f32::detail;
@@ -738,5 +746,17 @@ hipError_t launch_box_dose(const float* hi, const float* lo, long n, double* out
     return hipGetLastError();
 }
 
-}  // namespace batch
+}  // namespace M1_BATCH_NS
+
+const BatchKernels& M1_BATCH_TABLE()
+{
+    using namespace M1_BATCH_NS;
+    static const BatchKernels k{F32_TX,         F32_TY,          F32_TZ,
+                                &launch_stage,  &launch_cfl,     &launch_boundary,
+                                &launch_mask,   &launch_compact, &launch_zero,
+                                &launch_scatter_dose, &launch_box_dose, &launch_features,
+                                &launch_labels, &launch_compare, &launch_slices,
+                                &launch_clear,  &stage_blocks_per_cu};
+    return k;
+}
 }  // namespace m1

@@ -38,8 +38,16 @@ namespace {
 std::string g_batch_create_error;
 }  // namespace
 
+namespace {
+// The batched kernel sets, one per compiled tile shape (the default first).
+#define M1_BATCH_TABLE_ENTRY(a, b, c) &M1_BATCH_TABLE_NAME(a, b, c),
+const BatchKernels& (*const kKernelSets[])() = {M1_BATCH_SHAPES(M1_BATCH_TABLE_ENTRY)};
+constexpr int kKernelSetCount = int(sizeof(kKernelSets) / sizeof(kKernelSets[0]));
+}  // namespace
+
 struct m1_batch {
     int device = 0;
+    const BatchKernels* K = &kKernelSets[0]();  // kernels of the tile shape in use
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -67,6 +75,7 @@ struct m1_batch {
     int* d_count = nullptr;  // LIST_HEADER ints: length, stage queues
     bool persistent = false; // stage kernels on a fixed grid with XCD queues
     int persistent_grid = 0; // resident workgroups of a stage kernel
+    int cus = 0;             // compute units of the device
     bool use_graphs = false; // capture plain steps as HIP graphs
     std::string graph_error; // why the last capture failed (empty: none)
     int64_t graph_steps = 0; // steps of the last march run as a graph
@@ -279,7 +288,8 @@ int m1_batch_create(m1_batch** out, int device)
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
                   "hipDeviceGetAttribute");
-        const int per_cu = batch::stage_blocks_per_cu();
+        c->cus = cus;
+        const int per_cu = c->K->blocks_per_cu();
         c->persistent_grid = std::max(XCD_QUEUES, (cus * per_cu) / XCD_QUEUES * XCD_QUEUES);
         dev_alloc(&c->d_gate, 1, "hipMalloc network gate");
         host_alloc(&c->h_gate, 1, "hipHostMalloc");
@@ -346,9 +356,9 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
             d.nx = q.nx;
             d.ny = q.ny;
             d.nz = q.nz;
-            d.ntx = (q.nx + F32_TX - 1) / F32_TX;
-            d.nty = (q.ny + F32_TY - 1) / F32_TY;
-            d.ntz = (q.nz + F32_TZ - 1) / F32_TZ;
+            d.ntx = (q.nx + c->K->tx - 1) / c->K->tx;
+            d.nty = (q.ny + c->K->ty - 1) / c->K->ty;
+            d.ntz = (q.nz + c->K->tz - 1) / c->K->tz;
             d.tile_off = int(tiles);
             d.pad = 0;
             d.cell_off = cells;
@@ -396,7 +406,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
                         dst[z] = uint8_t(src[z]);
                         box_count[size_t(src[z])] += 1;
                         const int t = d.tile_off +
-                                      ((x / F32_TX) * d.nty + y / F32_TY) * d.ntz + z / F32_TZ;
+                                      ((x / c->K->tx) * d.nty + y / c->K->ty) * d.ntz + z / c->K->tz;
                         tile_mats[size_t(t)] |= uint8_t(1u << src[z]);
                         tile_rho[size_t(t)] += float(mats[size_t(src[z])].density);
                         tile_cells[size_t(t)] += 1;
@@ -566,7 +576,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                               c->dose_hi, c->dose_lo, {c->ck_U[0], c->ck_U[1]},
                               {c->ck_hi[0], c->ck_hi[1]}, {c->ck_lo[0], c->ck_lo[1]},
                               c->jf_hi,  c->jf_lo, c->js_hi, c->js_lo};
-            hip_check(batch::launch_slices(sa, int(c->max_box_cells), s), "judge slice kernel");
+            hip_check(c->K->slices(sa, int(c->max_box_cells), s), "judge slice kernel");
         };
         if (c->recording)
             hip_check(hipMemsetAsync(c->d_rcount, 0, sizeof(int), s), "sample count reset");
@@ -590,9 +600,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             BatchMaskArgs ma{c->d_box,      c->d_step, c->d_tile_box, int(T), c->d_tile_max,
                              c->d_tile_face, c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1,
                              nullptr,       nullptr,   nullptr,   nullptr};
-            hip_check(batch::launch_mask(ma, s), "tile mask kernel");
+            hip_check(c->K->mask(ma, s), "tile mask kernel");
             cur = 1 - cur;
-            hip_check(batch::launch_compact(c->d_flags[cur], int(T), c->d_list, c->d_count, s),
+            hip_check(c->K->compact(c->d_flags[cur], int(T), c->d_list, c->d_count, s),
                       "tile compaction kernel");
         }
         const size_t nm = c->mats.size();
@@ -649,7 +659,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 a.dose_hi = a.dose_lo = nullptr;
                 a.bnd_stage = 0;
                 Timed t(c->prof, s, K_STAGE0);
-                hip_check(batch::launch_stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
+                hip_check(c->K->stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
             }
             {
                 persist(1);
@@ -657,7 +667,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 a.out = c->V2;
                 a.bnd_stage = 1;
                 Timed t(c->prof, s, K_STAGE1);
-                hip_check(batch::launch_stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
+                hip_check(c->K->stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
             }
             {
                 persist(2);
@@ -667,7 +677,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 a.dose_lo = dlo;
                 a.bnd_stage = 2;
                 Timed t(c->prof, s, K_STAGE2);
-                hip_check(batch::launch_stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
+                hip_check(c->K->stage(STAGE2, ifac, a, s), "stage 2 kernel (batch)");
             }
         };
         // Step coefficients and boundary traces of the step about to run.
@@ -681,7 +691,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             BatchBoundaryArgs ba{c->d_box, c->d_step, B,       c->spatial,
                                  c->ids,   c->d_coef, c->bnd, ifac ? 1 : 0};
             Timed t(c->prof, s, K_BOUNDARY);
-            hip_check(batch::launch_boundary(ba, c->max_cols, s), "boundary kernel (batch)");
+            hip_check(c->K->boundary(ba, c->max_cols, s), "boundary kernel (batch)");
         };
         // Next step's tile set from the state after this step (computed set
         // flags[k], next set flags[1 - k]); the network runs when gate (device)
@@ -699,28 +709,28 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                     c->decision,   thr,           c->d_pred,      c->horizon,
                                     gate};
                 Timed t(c->prof, s, K_MODEL);
-                hip_check(batch::launch_features(fa, s), "tile-skip network kernel");
+                hip_check(c->K->features(fa, s), "tile-skip network kernel");
             }
             {
                 BatchMaskArgs ma{c->d_box,      c->d_step,       c->d_tile_box,
                                  int(T),        c->d_tile_max,   c->d_tile_face, c->d_flags[k],
                                  c->d_flags[1 - k], thr,
-                                 tile_radius(F32_TX), tile_radius(F32_TY), tile_radius(F32_TZ),
+                                 tile_radius(c->K->tx), tile_radius(c->K->ty), tile_radius(c->K->tz),
                                  skipping ? 0 : 1,
                                  use_model ? c->d_pred : nullptr,
                                  c->recording ? c->d_ring : nullptr,
                                  c->recording ? c->d_wmax : nullptr,
                                  nullptr};
                 Timed t(c->prof, s, K_TILE_MASK);
-                hip_check(batch::launch_mask(ma, s), "tile mask kernel (batch)");
-                hip_check(batch::launch_compact(c->d_flags[1 - k], int(T), c->d_list, c->d_count,
+                hip_check(c->K->mask(ma, s), "tile mask kernel (batch)");
+                hip_check(c->K->compact(c->d_flags[1 - k], int(T), c->d_list, c->d_count,
                                                 s),
                           "tile compaction kernel");
             }
             BatchZeroArgs za{c->d_box,         c->d_tile_box, int(T), c->d_flags[k],
                              c->d_flags[1 - k], c->U,         c->V1,  c->V2};
             Timed t(c->prof, s, K_TILE_ZERO);
-            hip_check(batch::launch_zero(za, s), "tile zero kernel (batch)");
+            hip_check(c->K->zero(za, s), "tile zero kernel (batch)");
         };
         // CFL maxima of every active box for the state in U over the tile set
         // flags[k], read back with the length of the list built from it.
@@ -737,14 +747,14 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, size_t(B) * sizeof(StepCoef),
                                      hipMemcpyHostToDevice, s),
                       "CFL coefficient upload");
-            hip_check(batch::launch_clear(c->d_red, B, s), "CFL reset");
+            hip_check(c->K->clear(c->d_red, B, s), "CFL reset");
             {
                 BatchCflArgs ca{c->d_box,  c->d_step,      c->d_tile_box, int(T), c->dx,
                                 c->dy,     c->dz,          c->U,          c->ids, c->spatial,
                                 c->d_flags[k], c->d_tile_mats, c->d_cfl, c->d_red, n.wave,
                                 vacuum_aware ? 1 : 0, thr};
                 Timed t(c->prof, s, K_CFL);
-                hip_check(batch::launch_cfl(ca, s), "CFL kernel (batch)");
+                hip_check(c->K->cfl(ca, s), "CFL kernel (batch)");
             }
             hip_check(hipMemcpyAsync(c->h_red, c->d_red, size_t(B) * sizeof(unsigned long long),
                                      hipMemcpyDeviceToHost, s),
@@ -954,8 +964,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 BatchMaskArgs fa{c->d_box,   c->d_step, c->d_tile_box, int(T), c->d_tile_max,
                                  c->d_tile_face, c->d_flags[cur], c->d_flags_full, 0.f, 0, 0, 0, 1,
                                  nullptr,      nullptr,   nullptr,       d_judge};
-                hip_check(batch::launch_mask(fa, s), "judge tile set");
-                hip_check(batch::launch_compact(c->d_flags_full, int(T), c->d_list_full,
+                hip_check(c->K->mask(fa, s), "judge tile set");
+                hip_check(c->K->compact(c->d_flags_full, int(T), c->d_list_full,
                                                 c->d_count_full, s),
                           "judge tile list");
                 c->prof.tiles_evaluated += n_full;
@@ -963,7 +973,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 slices(SLICE_RESTORE, d_save);
                 BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags_full,
                                  c->d_flags[cur], c->U, c->V1, c->V2};
-                hip_check(batch::launch_zero(za, s), "judge tile reset");
+                hip_check(c->K->zero(za, s), "judge tile reset");
                 a.judge = d_judge;
                 a.judge_hi = c->js_hi;
                 a.judge_lo = c->js_lo;
@@ -976,7 +986,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                           "judge reset");
                 BatchCompareArgs ca{c->d_box, d_judge,  B,         c->jf_hi,  c->jf_lo,
                                     c->js_hi, c->js_lo, c->d_jdiff, c->d_jfull};
-                hip_check(batch::launch_compare(ca, int(c->max_box_cells), s), "judge compare");
+                hip_check(c->K->compare(ca, int(c->max_box_cells), s), "judge compare");
                 hip_check(hipMemcpyAsync(c->h_jdiff, c->d_jdiff, nb * sizeof(unsigned),
                                          hipMemcpyDeviceToHost, s),
                           "judge read-back");
@@ -1002,14 +1012,14 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 if (iter >= c->record_horizon) {
                     BatchLabelArgs la{int(T),     c->d_cand, c->d_feat, c->d_wmax, thr,
                                       c->d_rx,    c->d_ry,   c->d_rcount, c->record_cap};
-                    hip_check(batch::launch_labels(la, s), "label kernel");
+                    hip_check(c->K->labels(la, s), "label kernel");
                 }
                 BatchFeatureArgs fa{c->d_box,    c->d_step,    c->d_tile_box,   int(T),
                                     c->d_flags[cur], c->d_tile_max, c->d_tile_sums, c->d_tile_face,
                                     c->d_tile_rho,
                                     c->d_feat,   nullptr,      nullptr,         0.f,
                                     thr,         nullptr,      0,               nullptr};
-                hip_check(batch::launch_features(fa, s), "feature kernel");
+                hip_check(c->K->features(fa, s), "feature kernel");
                 hip_check(hipMemcpyAsync(c->d_cand, c->d_ring, T, hipMemcpyDeviceToDevice, s),
                           "candidate save");
                 hip_check(hipMemsetAsync(c->d_wmax, 0, T * sizeof(float), s), "window reset");
@@ -1031,7 +1041,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 for (int b = 0; b < B; ++b) {
                     const m1_beamlet& q = c->bl[size_t(b)];
                     const BoxDesc& d = c->boxes[size_t(b)];
-                    hip_check(batch::launch_scatter_dose(c->dose_hi + d.cell_off,
+                    hip_check(c->K->scatter_dose(c->dose_hi + d.cell_off,
                                                          c->dose_lo + d.cell_off, q.nx, q.ny, q.nz,
                                                          q.x0, q.y0, c->py, c->pz, c->d_phantom, s),
                               "dose scatter kernel");
@@ -1061,7 +1071,7 @@ int m1_batch_beamlet_dose(m1_batch* c, int32_t beamlet, double* dose)
             throw ValueError{"no march has run on this batch"};
         const BoxDesc& d = c->boxes[size_t(beamlet)];
         const long n = long(d.nx) * d.ny * d.nz;
-        hip_check(batch::launch_box_dose(c->dose_hi + d.cell_off, c->dose_lo + d.cell_off, n,
+        hip_check(c->K->box_dose(c->dose_hi + d.cell_off, c->dose_lo + d.cell_off, n,
                                          c->d_box_dose, c->stream),
                   "beamlet dose kernel");
         hip_check(hipMemcpyAsync(dose, c->d_box_dose, size_t(n) * sizeof(double),
@@ -1116,6 +1126,35 @@ int m1_batch_host_ms(m1_batch* c, double* setup_ms)
     return guarded(c, [&] {
         if (setup_ms)
             *setup_ms = c->host_ms;
+    });
+}
+
+int32_t m1_batch_tile_shape_count(void) { return kKernelSetCount; }
+
+int m1_batch_tile_shape(int32_t index, int32_t* tx, int32_t* ty, int32_t* tz)
+{
+    if (index < 0 || index >= kKernelSetCount)
+        return M1_ERR_VALUE;
+    const BatchKernels& k = kKernelSets[index]();
+    if (tx)
+        *tx = k.tx;
+    if (ty)
+        *ty = k.ty;
+    if (tz)
+        *tz = k.tz;
+    return M1_OK;
+}
+
+int m1_batch_set_tile_shape(m1_batch* c, int32_t index)
+{
+    return guarded(c, [&] {
+        if (index < 0 || index >= kKernelSetCount)
+            throw ValueError{"tile shape index out of range"};
+        if (c->have_problem)
+            throw ValueError{"set the tile shape before m1_batch_set_problem"};
+        c->K = &kKernelSets[index]();
+        c->persistent_grid =
+            std::max(XCD_QUEUES, (c->cus * c->K->blocks_per_cu()) / XCD_QUEUES * XCD_QUEUES);
     });
 }
 

@@ -135,11 +135,16 @@ struct StageArgs32 {
     int f64_closure;        // face projection, closure, eigenvalues, HLL in double
 };
 
-// Workgroup tile geometry of the FP32 stage kernels (cells per tile in x, y, z).
+// Workgroup tile geometry of the FP32 stage kernels (cells per tile in x, y,
+// z) of this translation unit; the batched kernels are compiled once per
+// shape of M1_BATCH_SHAPES.
+#ifndef M1_TILE_TX
+#define M1_TILE_TX 16
+#endif
 #ifndef M1_TILE_TY
 #define M1_TILE_TY 16
 #endif
-constexpr int F32_TX = 16, F32_TY = M1_TILE_TY, F32_TZ = 256 / M1_TILE_TY;
+constexpr int F32_TX = M1_TILE_TX, F32_TY = M1_TILE_TY, F32_TZ = 256 / M1_TILE_TY;
 // Tile-skip dilation: a step's three stages reach 3 x 3 = 9 cells (the WENO-Z
 // stencil half-width per stage), so tiles within 9 cells of a tile holding
 // phi above the threshold are evaluated.
@@ -412,28 +417,38 @@ struct BatchZeroArgs {
     float* V2;
 };
 
-namespace batch {
-hipError_t launch_stage(int kind, bool integrating_factor, const BatchStageArgs& a,
+// Launchers of the batched kernels compiled for one tile shape.
+struct BatchKernels {
+    int tx, ty, tz;  // cells per tile
+    hipError_t (*stage)(int kind, bool integrating_factor, const BatchStageArgs& a,
                         hipStream_t s);
-hipError_t launch_cfl(const BatchCflArgs& a, hipStream_t s);
-hipError_t launch_boundary(const BatchBoundaryArgs& a, int max_columns, hipStream_t s);
-hipError_t launch_mask(const BatchMaskArgs& a, hipStream_t s);
-// list = ascending indices i < n with flags[i] != 0; *count = their number
-hipError_t launch_compact(const uint8_t* flags, int n, int* list, int* count, hipStream_t s);
-hipError_t launch_zero(const BatchZeroArgs& a, hipStream_t s);
-// phantom[x0 + x, y0 + y, z] += hi + lo over the box (px, py, pz phantom shape)
-hipError_t launch_scatter_dose(const float* hi, const float* lo, int nx, int ny, int nz, int x0,
+    hipError_t (*cfl)(const BatchCflArgs& a, hipStream_t s);
+    hipError_t (*boundary)(const BatchBoundaryArgs& a, int max_columns, hipStream_t s);
+    hipError_t (*mask)(const BatchMaskArgs& a, hipStream_t s);
+    // list = ascending indices i < n with flags[i] != 0; count[0] = their
+    // number, count[1 ..] (the stage queues) = 0
+    hipError_t (*compact)(const uint8_t* flags, int n, int* list, int* count, hipStream_t s);
+    hipError_t (*zero)(const BatchZeroArgs& a, hipStream_t s);
+    // phantom[x0 + x, y0 + y, z] += hi + lo over the box (px, py, pz phantom shape)
+    hipError_t (*scatter_dose)(const float* hi, const float* lo, int nx, int ny, int nz, int x0,
                                int y0, int py, int pz, double* phantom, hipStream_t s);
-hipError_t launch_box_dose(const float* hi, const float* lo, long n, double* out, hipStream_t s);
-hipError_t launch_features(const BatchFeatureArgs& a, hipStream_t s);
-hipError_t launch_labels(const BatchLabelArgs& a, hipStream_t s);
-hipError_t launch_compare(const BatchCompareArgs& a, int max_cells, hipStream_t s);
-hipError_t launch_slices(const BatchSliceArgs& a, int max_cells, hipStream_t s);
-// p[0 .. n) = 0 (a kernel, so that graphs hold no memset node).
-hipError_t launch_clear(unsigned long long* p, int n, hipStream_t s);
-// Workgroups of the stage kernels resident per CU (occupancy).
-int stage_blocks_per_cu();
-}  // namespace batch
+    hipError_t (*box_dose)(const float* hi, const float* lo, long n, double* out, hipStream_t s);
+    hipError_t (*features)(const BatchFeatureArgs& a, hipStream_t s);
+    hipError_t (*labels)(const BatchLabelArgs& a, hipStream_t s);
+    hipError_t (*compare)(const BatchCompareArgs& a, int max_cells, hipStream_t s);
+    hipError_t (*slices)(const BatchSliceArgs& a, int max_cells, hipStream_t s);
+    // p[0 .. n) = 0 (a kernel, so that graphs hold no memset node)
+    hipError_t (*clear)(unsigned long long* p, int n, hipStream_t s);
+    // stage-kernel workgroups resident per CU (occupancy)
+    int (*blocks_per_cu)();
+};
+
+// Tile shapes (x, y, z cells; y * z = 256 threads, x * y <= 256) the batched
+// kernels are compiled for, the default first.  The build reads this list.
+#define M1_BATCH_SHAPES(X) X(16, 16, 16) X(16, 8, 32) X(8, 16, 16) X(32, 8, 32)
+#define M1_BATCH_TABLE_NAME(a, b, c) batch_kernels_##a##x##b##x##c
+#define M1_DECLARE_BATCH_TABLE(a, b, c) const BatchKernels& M1_BATCH_TABLE_NAME(a, b, c)();
+M1_BATCH_SHAPES(M1_DECLARE_BATCH_TABLE)
 
 namespace f32 {
 hipError_t launch_stage(int kind, bool integrating_factor, const StageArgs32& a, hipStream_t s);

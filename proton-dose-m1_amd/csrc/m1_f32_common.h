@@ -54,6 +54,7 @@ constexpr int NT = TY * TZ;
 constexpr int PY = TY + 6;
 constexpr int PZ = TZ + 6;
 constexpr int NHALO = 6 * TZ + 6 * TY;
+static_assert(NHALO <= NT, "one y-z halo cell per thread");
 constexpr double REAL_EPS = 1.0e-10;  // weno_z.py:10, realizability.py:182
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -109,6 +109,10 @@ SIGNATURES = [
     ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
     ("m1_batch_set_persistent", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_tile_shape_count", C.c_int32, []),
+    ("m1_batch_tile_shape", C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]),
+    ("m1_batch_set_tile_shape", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_set_graphs", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_graph_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
                                       C.POINTER(C.c_char_p)]),
@@ -372,6 +376,19 @@ def tile_shape():
     return tuple(x.value for x in v)
 
 
+def batch_tile_shapes():
+    """Tile shapes (x, y, z cells) the batched kernels are compiled for; index
+    0 is the default."""
+    lib = load_library()
+    out = []
+    for i in range(lib.m1_batch_tile_shape_count()):
+        v = [C.c_int32() for _ in range(3)]
+        if lib.m1_batch_tile_shape(i, *(C.byref(x) for x in v)) != M1_OK:
+            raise MarchError("tile shape query failed")
+        out.append(tuple(x.value for x in v))
+    return out
+
+
 class Batch:
     """One libm1march batch: a phantom and its beamlet boxes in device memory."""
 
@@ -385,6 +402,7 @@ class Batch:
         self.shape = None
         self.boxes = ()
         self.model = None
+        self.tile_shape = batch_tile_shapes()[0]
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -479,6 +497,12 @@ class Batch:
             self._check(self.lib.m1_batch_kernel_stats(self.h, k, C.byref(n), C.byref(ms)))
             out[self.lib.m1_kernel_name(k).decode()] = (n.value, ms.value)
         return out
+
+    def set_tile_shape(self, index: int):
+        """Use the kernels of tile shape `index` (batch_tile_shapes()); call
+        before set_problem."""
+        self._check(self.lib.m1_batch_set_tile_shape(self.h, int(index)))
+        self.tile_shape = batch_tile_shapes()[int(index)]
 
     def set_persistent(self, enable: bool):
         """Persistent stage kernels with per-XCD tile queues (True) or one

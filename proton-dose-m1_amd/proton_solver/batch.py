@@ -275,13 +275,16 @@ class BatchResult:
         return self.handle.beamlet_dose(b)
 
 
-def prepare_batch(phantom: Phantom, beamlets, config, boxes=None, device=None):
-    """Bind a phantom and its beamlets to a device batch (returns it with the boxes)."""
+def prepare_batch(phantom: Phantom, beamlets, config, boxes=None, device=None, tile_shape=0):
+    """Bind a phantom and its beamlets to a device batch (returns it with the
+    boxes); tile_shape indexes _native.batch_tile_shapes()."""
     n = config.numerics
     if boxes is None:
         boxes = beamlet_boxes(phantom, beamlets, n.energy_min, n.energy_window_sigma)
     dev = config.device.device if device is None else device
     handle = _native.Batch(dev)
+    if tile_shape:
+        handle.set_tile_shape(tile_shape)
     profiles, beams = [], []
     for box, bl in zip(boxes, beamlets):
         _, _, inflow, beam = box_problem(phantom, box, bl)

@@ -150,3 +150,27 @@ def test_box_inflow_is_the_full_phantom_inflow_restricted_to_the_box():
     sub = np.asarray(inflow_f.spatial)[x0:x0 + nx, y0:y0 + ny]
     assert np.asarray(inflow_f.spatial).sum() == pytest.approx(sub.sum(), rel=1e-14)
     np.testing.assert_allclose(np.asarray(inflow_b.spatial), sub, rtol=1e-9, atol=0)
+
+
+def test_batch_tile_shapes_listed_by_the_library():
+    """The library reports the tile shapes its batched kernels were built for
+    (no GPU call); the default is 16^3 and every shape maps 256 threads."""
+    from proton_solver import _native
+
+    shapes = _native.batch_tile_shapes()
+    assert shapes[0] == (16, 16, 16) and len(shapes) >= 2
+    assert len(set(shapes)) == len(shapes)
+    for tx, ty, tz in shapes:
+        assert ty * tz == 256 and tx * ty <= 256 and min(tx, ty, tz) >= 8
+
+
+def test_autotune_sample_spans_the_energies():
+    from proton_solver.autotune import sample_beamlets
+    from proton_solver.batch import random_beamlets
+
+    bl = random_beamlets(np.random.default_rng(3), 40)
+    sub = sample_beamlets(bl, 6)
+    energies = sorted(b.energy for b in bl)
+    got = sorted(b.energy for b in sub)
+    assert len(sub) == 6 and got[0] == energies[0] and got[-1] == energies[-1]
+    assert sample_beamlets(bl, 100) and len(sample_beamlets(bl, 100)) == 40

@@ -241,3 +241,52 @@ def test_graph_steps_are_bitwise_the_launched_steps():
     assert err == "" and n_graph > 0.9 * int(s1.max()) and n_timed == 0
     assert np.array_equal(s0, s1) and np.array_equal(s0, s2)
     assert np.array_equal(d0, d1) and np.array_equal(d0, d2)
+
+
+@pytest.mark.parametrize("shape", range(1, len(_native.batch_tile_shapes())))
+def test_every_tile_shape_gives_the_same_march(shape):
+    """The kernels of each compiled tile shape evaluate the same arithmetic on
+    a different tile set: the dose agrees with the default shape to the
+    sub-threshold content the sets differ by, and against the FP64 march
+    within the acceptance bounds."""
+    ph, beamlets, boxes = small_workload()
+    cfg = _Cfg()
+    num = _native.numerics_struct(cfg)
+    h0, _ = prepare_batch(ph, beamlets, cfg, boxes)
+    d0, s0 = h0.march(num)
+    h, _ = prepare_batch(ph, beamlets, cfg, boxes, tile_shape=shape)
+    h.reset_stats()
+    d1, s1 = h.march(num)
+    ev, full = h.tile_stats()
+    err = metrics.max_abs_difference_over_max(d1, d0)
+    md = metrics.maximum_dose_relative_difference(d1, d0)
+    say("tile shape", h.tile_shape, "tiles", ev, "of", full, "max|dD|/maxD vs 16^3", err,
+        "max-dose", md, "steps", s1.tolist(), s0.tolist())
+    assert h.tile_shape == _native.batch_tile_shapes()[shape]
+    assert err < 2e-3 and abs(md) < 1.5e-3
+    # the per-tile and persistent launches of this shape agree bit for bit
+    h.set_persistent(True)
+    d2, s2 = h.march(num)
+    assert np.array_equal(s1, s2) and np.array_equal(d1, d2)
+
+
+def test_tile_shape_must_precede_the_problem():
+    ph, beamlets, boxes = small_workload()
+    h, _ = prepare_batch(ph, beamlets, _Cfg(), boxes)
+    with pytest.raises(ValueError, match="before"):
+        h.set_tile_shape(1)
+    with pytest.raises(ValueError, match="out of range"):
+        _native.Batch(0).set_tile_shape(99)
+
+
+def test_autotune_picks_a_timed_candidate():
+    from proton_solver.autotune import autotune
+
+    ph, beamlets, boxes = small_workload()
+    choice = autotune(ph, beamlets, _Cfg(), sample=3)
+    say("autotune", choice.shape, choice.mode, choice.seconds, choice.table)
+    shapes = _native.batch_tile_shapes()
+    assert len(choice.table) == len(shapes) + 2
+    assert choice.shape == shapes[choice.shape_index]
+    assert choice.seconds == min(r["seconds"] for r in choice.table) or \
+        abs(choice.seconds - min(r["seconds"] for r in choice.table)) < 1e-3
