@@ -87,6 +87,47 @@ __device__ __forceinline__ float weno_left(float v0, float v1, float v2, float v
     return fmaf(a0, p0, fmaf(a1, p1, a2 * p2)) * rcpf(a0 + a1 + a2);
 }
 
+// Both WENO-Z5 states of one cell from its stencil v0..v4 (centre v2): the
+// left state at its upper face, weno_left(v0..v4), and the right state at its
+// lower face, weno_left(v4..v0).  The mirrored stencil's smoothness
+// indicators are the same three in reverse order, so the indicators, tau5
+// and the ratios are computed once; each side keeps its own candidate
+// polynomials and linear weights.
+__device__ __forceinline__ void weno_pair(float v0, float v1, float v2, float v3, float v4,
+                                          float eps, float& up, float& lo)
+{
+    float a = fmaf(-2.f, v1, v0) + v2;
+    float b = fmaf(3.f, v2, fmaf(-4.f, v1, v0));
+    const float b0 = fmaf((13.f / 12.f) * a, a, 0.25f * (b * b));
+    a = fmaf(-2.f, v2, v1) + v3;
+    b = v1 - v3;
+    const float b1 = fmaf((13.f / 12.f) * a, a, 0.25f * (b * b));
+    a = fmaf(-2.f, v3, v2) + v4;
+    b = fmaf(-4.f, v3, fmaf(3.f, v2, v4));
+    const float b2 = fmaf((13.f / 12.f) * a, a, 0.25f * (b * b));
+    const float tau = fabsf(b2 - b0);
+    const float r0 = fminf(tau * rcpf(b0 + eps), 1e18f);
+    const float r1 = fminf(tau * rcpf(b1 + eps), 1e18f);
+    const float r2 = fminf(tau * rcpf(b2 + eps), 1e18f);
+    const float a1 = fmaf(0.6f * r1, r1, 0.6f);
+    {
+        const float p0 = fmaf(v0, 1.f / 3.f, fmaf(v1, -7.f / 6.f, v2 * (11.f / 6.f)));
+        const float p1 = fmaf(v2, 5.f / 6.f, fmaf(v1, -1.f / 6.f, v3 * (1.f / 3.f)));
+        const float p2 = fmaf(v2, 1.f / 3.f, fmaf(v3, 5.f / 6.f, v4 * (-1.f / 6.f)));
+        const float a0 = fmaf(0.1f * r0, r0, 0.1f);
+        const float a2 = fmaf(0.3f * r2, r2, 0.3f);
+        up = fmaf(a0, p0, fmaf(a1, p1, a2 * p2)) * rcpf(a0 + a1 + a2);
+    }
+    {
+        const float p0 = fmaf(v4, 1.f / 3.f, fmaf(v3, -7.f / 6.f, v2 * (11.f / 6.f)));
+        const float p1 = fmaf(v2, 5.f / 6.f, fmaf(v3, -1.f / 6.f, v1 * (1.f / 3.f)));
+        const float p2 = fmaf(v2, 1.f / 3.f, fmaf(v1, 5.f / 6.f, v0 * (-1.f / 6.f)));
+        const float a0 = fmaf(0.1f * r2, r2, 0.1f);
+        const float a2 = fmaf(0.3f * r0, r0, 0.3f);
+        lo = fmaf(a0, p0, fmaf(a1, p1, a2 * p2)) * rcpf(a0 + a1 + a2);
+    }
+}
+
 // Realizability projection in double (realizability.py:132-176).
 __device__ __forceinline__ void project(double& p, double& x, double& y, double& z, double& jn)
 {
@@ -350,19 +391,19 @@ __device__ __forceinline__ void hll_f(const FaceF& cl, const FaceF& cr, int wave
         F[c] = (sr * FL[c] - sl * FR[c] + sp * (Rp[c] - Lp[c])) * inv;
 }
 
-// Face flux from six cell states s[c][0..5] (cells f-3..f+2 of face f-1/2).
+// HLL flux of a face from its reconstructed left and right states.
 // D: projection, closure, eigenvalues and HLL in double; otherwise in float
 // with the cancellation-free closure above.
 template <int AXIS, bool D>
-__device__ __forceinline__ void face_flux(const float s[4][6], int wave, float weps, double eeps,
-                                          float F[4])
+__device__ __forceinline__ void face_hll(const float Ls[4], const float Rs[4], int wave,
+                                         double eeps, float F[4])
 {
     if constexpr (D) {
         double L[4], R[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            L[c] = weno_left(s[c][0], s[c][1], s[c][2], s[c][3], s[c][4], weps);
-            R[c] = weno_left(s[c][5], s[c][4], s[c][3], s[c][2], s[c][1], weps);
+            L[c] = Ls[c];
+            R[c] = Rs[c];
         }
         double jl, jr;
         project(L[0], L[1], L[2], L[3], jl);
@@ -386,26 +427,20 @@ __device__ __forceinline__ void face_flux(const float s[4][6], int wave, float w
         for (int c = 0; c < 4; ++c)
             F[c] = float((sr * FL[c] - sl * FR[c] + sp * (R[c] - L[c])) * inv);
     } else {
-        float L[4], R[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            L[c] = weno_left(s[c][0], s[c][1], s[c][2], s[c][3], s[c][4], weps);
-            R[c] = weno_left(s[c][5], s[c][4], s[c][3], s[c][2], s[c][1], weps);
-        }
-        const FaceF cl = face_state(L[0], L[1], L[2], L[3]);
-        const FaceF cr = face_state(R[0], R[1], R[2], R[3]);
+        const FaceF cl = face_state(Ls[0], Ls[1], Ls[2], Ls[3]);
+        const FaceF cr = face_state(Rs[0], Rs[1], Rs[2], Rs[3]);
         hll_f<AXIS>(cl, cr, wave, float(eeps), F);
     }
 }
 
 template <bool D>
-__device__ __forceinline__ void face_flux_yz(bool zaxis, const float s[4][6], int wave,
-                                             float weps, double eeps, float F[4])
+__device__ __forceinline__ void face_hll_yz(bool zaxis, const float L[4], const float R[4],
+                                            int wave, double eeps, float F[4])
 {
     if (zaxis)
-        face_flux<2, D>(s, wave, weps, eeps, F);
+        face_hll<2, D>(L, R, wave, eeps, F);
     else
-        face_flux<1, D>(s, wave, weps, eeps, F);
+        face_hll<1, D>(L, R, wave, eeps, F);
 }
 
 // Stencil input of one cell: U directly, or proj(C_k V_k / S_k) for stages 1, 2.
@@ -624,8 +659,15 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 {
     __shared__ StepCoef32 sc;
     __shared__ float P[4][PY][PZ];
+    // Fy / Fz hold the upper WENO states of the faces, then their fluxes;
+    // Hy / Hz the lower states of the cells just above the tile.
     __shared__ float Fy[4][TY + 1][TZ];
     __shared__ float Fz[4][TY][TZ + 1];
+    __shared__ float Hy[4][TZ];
+    __shared__ float Hz[4][TY];
+    // per thread: lower states of its cell along z and y (until the fluxes)
+    // and the upper x state carried to the next plane
+    __shared__ float Rzs[4][TY][TZ], Rys[4][TY][TZ], Lxs[4][TY][TZ];
     __shared__ float wmax[3][NT / 64];
     __shared__ float plane[NT / 64][TX];  // per wave, max phi of each x plane
     __shared__ float colmax[TY][TZ];      // per (y, z) column, max phi over x
@@ -671,25 +713,36 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     const int hgz_raw = z0 + hz - 3;
     const int hgz = min(max(hgz_raw, 0), nz - 1);
 
+    // x: a window of six planes of this thread's column; each cell's two
+    // WENO states are computed once, the left state of its upper face
+    // carried to the next plane.
     float W[6][4];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
         load_state<KIND, D>(a, sc, cidx(clampx(x0 - 3 + k), yc, zc), N, W[k]);
     float Flo[4];
     {
-        float s[4][6];
+        float L[4], R[4], up;
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-                s[c][k] = W[k][c];
-        face_flux<0, D>(s, wave, weps, eeps, Flo);
+        for (int c = 0; c < 4; ++c) {
+            L[c] = weno_left(W[0][c], W[1][c], W[2][c], W[3][c], W[4][c], weps);
+            weno_pair(W[1][c], W[2][c], W[3][c], W[4][c], W[5][c], weps, up, R[c]);
+            Lxs[c][ty][tz] = up;
+        }
+        face_hll<0, D>(L, R, wave, eeps, Flo);
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
             W[k][c] = W[k + 1][c];
+
+    // y and z: lanes of the halo cells' states next to the tile (upper state
+    // of the cell below the tile, lower state of the cell above it).
+    const int nh = 2 * (TY + TZ);
+    const bool hz_lane = tid < 2 * TY;
+    const int h_line = hz_lane ? tid % TY : (tid - 2 * TY) % TZ;
+    const bool h_high = hz_lane ? tid >= TY : tid - 2 * TY >= TZ;
 
     for (int x = x0; x < x1; ++x) {
         load_state<KIND, D>(a, sc, cidx(clampx(x + 3), yc, zc), N, W[5]);
@@ -718,53 +771,95 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
         }
         float Fhi[4];
         {
-            float s[4][6];
+            float L[4], R[4], up;
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = W[k][c];
-            face_flux<0, D>(s, wave, weps, eeps, Fhi);
+            for (int c = 0; c < 4; ++c) {
+                L[c] = Lxs[c][ty][tz];
+                weno_pair(W[1][c], W[2][c], W[3][c], W[4][c], W[5][c], weps, up, R[c]);
+                Lxs[c][ty][tz] = up;
+            }
+            face_hll<0, D>(L, R, wave, eeps, Fhi);
         }
         __syncthreads();
+        // This cell's states along z and y; the upper states go to LDS (the
+        // flux arrays, slot of the face they belong to) for the cell above.
+#pragma unroll 1
+        for (int c = 0; c < 4; ++c) {
+            float up, lo;
+            weno_pair(P[c][ty + 3][tz + 1], P[c][ty + 3][tz + 2], P[c][ty + 3][tz + 3],
+                      P[c][ty + 3][tz + 4], P[c][ty + 3][tz + 5], weps, up, lo);
+            Fz[c][ty][tz + 1] = up;
+            Rzs[c][ty][tz] = lo;
+            weno_pair(P[c][ty + 1][tz + 3], P[c][ty + 2][tz + 3], P[c][ty + 3][tz + 3],
+                      P[c][ty + 4][tz + 3], P[c][ty + 5][tz + 3], weps, up, lo);
+            Fy[c][ty + 1][tz] = up;
+            Rys[c][ty][tz] = lo;
+        }
+        if (tid < nh) {
+            // z lanes: row h_line, centre z = -1 (upper state) or TZ (lower);
+            // y lanes: column h_line, centre y = -1 or TY
+#pragma unroll 1
+            for (int c = 0; c < 4; ++c) {
+                float v[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                    v[k] = hz_lane ? P[c][h_line + 3][(h_high ? TZ + 1 : 0) + k]
+                                   : P[c][(h_high ? TY + 1 : 0) + k][h_line + 3];
+                const float st = h_high ? weno_left(v[4], v[3], v[2], v[1], v[0], weps)
+                                        : weno_left(v[0], v[1], v[2], v[3], v[4], weps);
+                if (hz_lane) {
+                    if (h_high)
+                        Hz[c][h_line] = st;
+                    else
+                        Fz[c][h_line][0] = st;
+                } else {
+                    if (h_high)
+                        Hy[c][h_line] = st;
+                    else
+                        Fy[c][0][h_line] = st;
+                }
+            }
+        }
+        __syncthreads();
+        // Fluxes: each cell's lower faces; the first TY + TZ lanes also the
+        // upper faces of the tile.
         {
-            float s[4][6], F[4];
+            float L[4], R[4], F[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = P[c][ty + 3][tz + k];
-            face_flux<2, D>(s, wave, weps, eeps, F);
+            for (int c = 0; c < 4; ++c) {
+                L[c] = Fz[c][ty][tz];
+                R[c] = Rzs[c][ty][tz];
+            }
+            face_hll<2, D>(L, R, wave, eeps, F);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 Fz[c][ty][tz] = F[c];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = P[c][ty + k][tz + 3];
-            face_flux<1, D>(s, wave, weps, eeps, F);
+            for (int c = 0; c < 4; ++c) {
+                L[c] = Fy[c][ty][tz];
+                R[c] = Rys[c][ty][tz];
+            }
+            face_hll<1, D>(L, R, wave, eeps, F);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 Fy[c][ty][tz] = F[c];
         }
         if (tid < TY + TZ) {
             const bool zaxis = tid < TY;
-            const int row = zaxis ? tid : 0;
-            const int col = zaxis ? 0 : tid - TY;
-            float s[4][6], F[4];
+            const int line = zaxis ? tid : tid - TY;
+            float L[4], R[4], F[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    s[c][k] = zaxis ? P[c][row + 3][TZ + k] : P[c][TY + k][col + 3];
-            face_flux_yz<D>(zaxis, s, wave, weps, eeps, F);
+            for (int c = 0; c < 4; ++c) {
+                L[c] = zaxis ? Fz[c][line][TZ] : Fy[c][TY][line];
+                R[c] = zaxis ? Hz[c][line] : Hy[c][line];
+            }
+            face_hll_yz<D>(zaxis, L, R, wave, eeps, F);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 if (zaxis)
-                    Fz[c][row][TZ] = F[c];
+                    Fz[c][line][TZ] = F[c];
                 else
-                    Fy[c][TY][col] = F[c];
+                    Fy[c][TY][line] = F[c];
             }
         }
         __syncthreads();
