@@ -54,10 +54,14 @@ STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
 #   stage 1: read V1 (16) + U (16) + ids (1), write V2 (16)
 #   stage 2: read V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16), dose pair RMW (16)
 STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 69}
-# HBM bytes per evaluated cell of the batched stage kernels, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes scaled by the same counters on a kernel of
-# known traffic (profiles/stage_kernels_batch_pmc_traffic.md).
-PMC_BYTES_PER_CELL_FP32 = {"stage0": 63.4, "stage1": 108.0, "stage2": 168.9}
+# HBM bytes per evaluated cell of the batched stage kernels per tile shape,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters
+# on a kernel of known traffic (profiles/stage_kernels_batch_pmc_traffic.md);
+# shapes without a measurement report traffic null.
+PMC_BYTES_PER_CELL_FP32 = {
+    (16, 16, 16): {"stage0": 58.6, "stage1": 84.6, "stage2": 105.5},
+    (8, 16, 16): {"stage0": 59.2, "stage1": 86.5, "stage2": 106.1},
+}
 
 
 def workload_config(precision):
@@ -186,7 +190,7 @@ def run_batch(args):
     # Tile shape and launch from the autotuner (a sample of the beamlets,
     # outside the timed region).
     tuned = None
-    if args.autotune:
+    if args.autotune and args.tile_shape is None:
         from proton_solver.autotune import autotune
 
         t0 = time.perf_counter()
@@ -201,7 +205,8 @@ def run_batch(args):
     boxes = beamlet_boxes(phantom, beamlets, cfg.numerics.energy_min,
                           cfg.numerics.energy_window_sigma)
     handle, boxes = prepare_batch(phantom, beamlets, cfg, boxes,
-                                  tile_shape=choice.shape_index if tuned else 0)
+                                  tile_shape=(choice.shape_index if tuned else
+                                              (args.tile_shape or 0)))
     if tuned:
         choice.apply(handle)
     setup_s = time.perf_counter() - t0
@@ -271,7 +276,8 @@ def run_batch(args):
     def cells_per_launch(launches):
         return ev * float(tx * ty * tz) * fill / launches
 
-    roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch, PMC_BYTES_PER_CELL_FP32)
+    roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch,
+                       PMC_BYTES_PER_CELL_FP32.get(tuple(handle.tile_shape)))
     if args.profile_calibration:
         # a kernel of known traffic for scaling the HBM counters: the beamlet
         # dose read-out reads a float pair and writes a double per box cell
@@ -397,7 +403,9 @@ def main():
                     help="skip the untimed march without the network")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="default tile shape and launch instead of the autotuner's choice")
-    ap.add_argument("--autotune-sample", type=int, default=12)
+    ap.add_argument("--autotune-sample", type=int, default=32)
+    ap.add_argument("--tile-shape", type=int, default=None,
+                    help="index of the batched kernels' tile shape (no autotuning)")
     ap.add_argument("--profile-calibration", action="store_true",
                     help="run a kernel of known HBM traffic after the timed march")
     args = ap.parse_args()
