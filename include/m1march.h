@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define M1_ABI_VERSION 4
+#define M1_ABI_VERSION 5
 
 enum {
     M1_OK = 0,
@@ -112,10 +112,11 @@ typedef struct {
      * tile.  No reference counterpart (the reference's closest analogue is
      * its moving active domain, moving_active_domain.py:107-193). */
     double tile_threshold;
-    /* Batched march with tile_threshold > 0 only (ignored elsewhere, as the
-     * reference's static energy_march ignores it): the vacuum-aware CFL of
-     * the reference's moving active domain (energy.py:459-537, 610-636,
-     * config.py:49).  A cell's axis speed counts only within 3 cells along
+    /* m1_march_moving: the vacuum-aware CFL of the moving active domain
+     * (energy.py:459-537, 610-636, config.py:49) at the truncation tolerance.
+     * Batched march with tile_threshold > 0: the same rule on the beamlet
+     * boxes (ignored by m1_march, as the reference's static energy_march
+     * ignores it).  A cell's axis speed counts only within 3 cells along
      * that axis of an active cell, active meaning normalised phi (or the z = 0
      * inflow phi) above tile_threshold; the reference's relative tolerance
      * is taken as the tile threshold, relative to the beamlet's peak inflow
@@ -170,6 +171,39 @@ int m1_march(m1_ctx *ctx, const m1_numerics *num,
              double *energies, int32_t energies_cap, int32_t *n_energies,
              const double *snapshot_energies, int32_t n_snapshots,
              double *snapshots, int32_t *snapshot_taken);
+
+/* _moving_active_energy_march (energy.py:1262-1494, moving_active_domain=
+ * True) with the box helpers of moving_active_domain.py:35-253.  The state
+ * set with m1_set_state (zero for the reference's U = None) is cut to the
+ * initial box (beam support widened by guard_cells, z in [0, 1 + guard_cells));
+ * before every step the box becomes the bounding box of the cells with
+ * phi > exp(-truncation_sigma^2 / 2) * max phi (the inflow phi included while
+ * the box touches z = 0) widened by guard_cells, the state is moved to it
+ * (zero where it is new), and the step runs on the box alone with the box's
+ * materials and an inflow that is zero once the box has left z = 0.  With
+ * num->vacuum_aware_cfl the energy step uses the vacuum-aware CFL at the
+ * same tolerance (m1_compute_energy_step_vacuum).  The dose is accumulated on
+ * the full grid (m1_get_dose); the final state, zero outside the last box,
+ * is left on the device (m1_get_state; store_final_state=False is the
+ * caller's choice not to read it).  energies, snapshots (full-grid states)
+ * and their arguments as m1_march.  boxes (nullable) receives for each of
+ * the first energies_cap steps the box {i0, i1, j0, j1, k0, k1} (half-open
+ * global cell ranges) the step was taken on.  FP64 and FP64_FAST only;
+ * guard_cells >= 4 and truncation_sigma > 0 (config.py:81-88). */
+int m1_march_moving(m1_ctx *ctx, const m1_numerics *num, int32_t guard_cells,
+                    double truncation_sigma, double *energies, int32_t energies_cap,
+                    int32_t *n_energies, const double *snapshot_energies,
+                    int32_t n_snapshots, double *snapshots, int32_t *snapshot_taken,
+                    int32_t *boxes);
+
+/* compute_energy_step with vacuum_relative_tolerance = relative_tolerance
+ * (energy.py:493-660) of the current device state on the whole grid: an axis's
+ * exact M1 speed counts in a cell only when a cell within 3 cells along that
+ * axis has phi > relative_tolerance * max(max phi, max inflow phi), the inflow
+ * phi counting at z = 0 (every cell counts when that maximum is zero).  With
+ * unit wave speeds it equals m1_compute_energy_step.  FP64 and FP64_FAST. */
+int m1_compute_energy_step_vacuum(m1_ctx *ctx, const m1_numerics *num, double energy,
+                                  double relative_tolerance, double *dE);
 
 /* spatial_operator (spatial.py:104-252, active_domain=False) of the current
  * device state at `energy`: rhs = -div F_HLL, (4,nx,ny,nz).  boundary_state

@@ -808,8 +808,69 @@ out:
 /* CFL and integrators                                                 */
 /* ------------------------------------------------------------------ */
 
-int or_compute_energy_step(const or_problem *p, const or_config *c,
-                           const double *U, double energy, double *dE_out)
+/* energy.py:459-490: grow a support mask by `radius` cells along one axis,
+ * one cell per pass. */
+static void dilate_axis(const uint8_t *in, uint8_t *out, uint8_t *tmp, int nx,
+                        int ny, int nz, int axis, int radius)
+{
+    long n = (long)nx * ny * nz;
+    long stride = axis == 0 ? (long)ny * nz : axis == 1 ? nz : 1;
+    int len = axis == 0 ? nx : axis == 1 ? ny : nz;
+    memcpy(out, in, (size_t)n);
+    for (int r = 0; r < radius; ++r) {
+        memcpy(tmp, out, (size_t)n);
+        for (long i = 0; i < n; ++i) {
+            int pos = (int)((i / stride) % len);
+            if (pos > 0 && tmp[i - stride])
+                out[i] = 1;
+            if (pos < len - 1 && tmp[i + stride])
+                out[i] = 1;
+        }
+    }
+}
+
+/* energy.py:493-537: directional activity masks of the vacuum-aware CFL.
+ * Uin is the (4, nx, ny) inflow state; masks is 3n bytes (x, y, z). */
+static int vacuum_masks(const or_problem *p, const double *U, const double *Uin,
+                        double tol, int radius, uint8_t *masks)
+{
+    int nx = p->nx, ny = p->ny, nz = p->nz;
+    long n = (long)nx * ny * nz, m = (long)nx * ny;
+    double mphi = U[0], min_ = Uin[0];
+    for (long i = 1; i < n; ++i)
+        if (U[i] > mphi)
+            mphi = U[i];
+    for (long i = 1; i < m; ++i)
+        if (Uin[i] > min_)
+            min_ = Uin[i];
+    double scale = min_ > mphi ? min_ : mphi;
+    if (scale == 0.0) {
+        memset(masks, 1, (size_t)(3 * n));
+        return 0;
+    }
+    double thr = tol * scale;
+    uint8_t *act = malloc((size_t)n), *tmp = malloc((size_t)n);
+    if (!act || !tmp) {
+        free(act);
+        free(tmp);
+        return fail("out of memory");
+    }
+    for (long i = 0; i < n; ++i)
+        act[i] = U[i] > thr;
+    for (long col = 0; col < m; ++col)
+        act[col * nz] |= Uin[col] > thr;
+    for (int axis = 0; axis < 3; ++axis)
+        dilate_axis(act, masks + axis * n, tmp, nx, ny, nz, axis, radius);
+    free(act);
+    free(tmp);
+    return 0;
+}
+
+/* energy.py:539-660.  vac_tol < 0 stands for vacuum_relative_tolerance=None;
+ * otherwise the directional speeds outside the WENO-Z stencil reach (3
+ * cells) of the active cells are dropped. */
+static int energy_step(const or_problem *p, const or_config *c, const double *U,
+                       double energy, double vac_tol, double *dE_out)
 {
     int nx = p->nx, ny = p->ny, nz = p->nz;
     long n = (long)nx * ny * nz, m = (long)nx * ny;
@@ -846,6 +907,17 @@ int or_compute_energy_step(const or_problem *p, const or_config *c,
             return -1;
         }
         or_eigenvalues(Uin, m, 2, li, li + m);
+        uint8_t *mask = NULL;
+        if (vac_tol >= 0.0) {
+            mask = malloc((size_t)(3 * n));
+            if (!mask || vacuum_masks(p, U, Uin, vac_tol, GHOST, mask)) {
+                free(mask);
+                free(lm);
+                free(Uin);
+                free(li);
+                return mask ? -1 : fail("out of memory");
+            }
+        }
 #pragma omp parallel for reduction(max : mx) schedule(static)
         for (long i = 0; i < n; ++i) {
             double ax = fmax(fabs(lm[i]), fabs(lm[n + i]));
@@ -856,11 +928,17 @@ int or_compute_energy_step(const or_problem *p, const or_config *c,
                 double ain = fmax(fabs(li[col]), fabs(li[m + col]));
                 az = fmax(az, ain);
             }
+            if (mask) {
+                ax = mask[i] ? ax : 0.0;
+                ay = mask[n + i] ? ay : 0.0;
+                az = mask[2 * n + i] ? az : 0.0;
+            }
             double s = S[p->ids[i]];
             double v = ax / (s * p->dx) + ay / (s * p->dy) + az / (s * p->dz);
             if (v > mx)
                 mx = v;
         }
+        free(mask);
         free(lm);
         free(Uin);
         free(li);
@@ -868,6 +946,21 @@ int or_compute_energy_step(const or_problem *p, const or_config *c,
     double dE = c->cfl / mx;
     *dE_out = dE <= c->max_energy_step ? dE : c->max_energy_step;
     return 0;
+}
+
+int or_compute_energy_step(const or_problem *p, const or_config *c,
+                           const double *U, double energy, double *dE_out)
+{
+    return energy_step(p, c, U, energy, -1.0, dE_out);
+}
+
+int or_compute_energy_step_vacuum(const or_problem *p, const or_config *c,
+                                  const double *U, double energy,
+                                  double relative_tolerance, double *dE_out)
+{
+    if (!(relative_tolerance >= 0.0))
+        return fail("relative tolerance must be non-negative");
+    return energy_step(p, c, U, energy, relative_tolerance, dE_out);
 }
 
 /* energy.py:663-722 */
@@ -1199,5 +1292,294 @@ out:
     free(targets);
     free(tmap);
     free(edep);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Moving active domain                                                */
+/* ------------------------------------------------------------------ */
+
+/* moving_active_domain.py:35-60: beam support + guard, z in [0, 1 + guard). */
+static int initial_box(const or_problem *p, int guard, int b[6])
+{
+    int ix0 = -1, ix1 = -1, iy0 = -1, iy1 = -1;
+    for (int i = 0; i < p->nx; ++i)
+        for (int j = 0; j < p->ny; ++j)
+            if (p->spatial[(long)i * p->ny + j] > 0.0) {
+                if (ix0 < 0)
+                    ix0 = i;
+                ix1 = i;
+                if (iy0 < 0 || j < iy0)
+                    iy0 = j;
+                if (j > iy1)
+                    iy1 = j;
+            }
+    if (ix0 < 0)
+        return fail("Beam inflow has empty transverse support.");
+    b[0] = ix0 - guard > 0 ? ix0 - guard : 0;
+    b[1] = ix1 + 1 + guard < p->nx ? ix1 + 1 + guard : p->nx;
+    b[2] = iy0 - guard > 0 ? iy0 - guard : 0;
+    b[3] = iy1 + 1 + guard < p->ny ? iy1 + 1 + guard : p->ny;
+    b[4] = 0;
+    b[5] = 1 + guard < p->nz ? 1 + guard : p->nz;
+    return 0;
+}
+
+/* moving_active_domain.py:63-104: the box's grid, material ids and inflow
+ * (the beam profile while the box touches z = 0, zero otherwise).  ids and
+ * spatial are caller buffers of the full grid's sizes. */
+static void local_problem(const or_problem *p, const int b[6], int32_t *ids,
+                          double *spatial, or_problem *lp)
+{
+    int nx = b[1] - b[0], ny = b[3] - b[2], nz = b[5] - b[4];
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            long g = ((long)(b[0] + i) * p->ny + (b[2] + j)) * p->nz + b[4];
+            long l = ((long)i * ny + j) * nz;
+            memcpy(ids + l, p->ids + g, sizeof(int32_t) * (size_t)nz);
+            spatial[(long)i * ny + j] =
+                b[4] == 0 ? p->spatial[(long)(b[0] + i) * p->ny + (b[2] + j)] : 0.0;
+        }
+    *lp = *p;
+    lp->nx = nx;
+    lp->ny = ny;
+    lp->nz = nz;
+    lp->ids = ids;
+    lp->spatial = spatial;
+}
+
+/* moving_active_domain.py:107-151: bounding box of phi > tol * scale (scale:
+ * the largest phi, and of the inflow phi while k0 == 0).  Returns 1 and the
+ * global box, or 0 for "no significant cell" (None). */
+static int significant_box(const or_problem *lp, const int b[6], const double *U,
+                           double energy, double tol, double *Uin, int out[6])
+{
+    long n = (long)lp->nx * lp->ny * lp->nz, m = (long)lp->nx * lp->ny;
+    int inflow = b[4] == 0;
+    double scale = U[0];
+    for (long i = 1; i < n; ++i)
+        if (U[i] > scale)
+            scale = U[i];
+    if (inflow) {
+        if (or_inflow_state(lp, energy, Uin))
+            return -1;
+        double mi = Uin[0];
+        for (long c = 1; c < m; ++c)
+            if (Uin[c] > mi)
+                mi = Uin[c];
+        if (mi > scale)
+            scale = mi;
+    }
+    if (scale == 0.0)
+        return 0;
+    double thr = tol * scale;
+    int lo[3] = {lp->nx, lp->ny, lp->nz}, hi[3] = {-1, -1, -1};
+    for (int i = 0; i < lp->nx; ++i)
+        for (int j = 0; j < lp->ny; ++j)
+            for (int k = 0; k < lp->nz; ++k) {
+                long l = ((long)i * lp->ny + j) * lp->nz + k;
+                int act = U[l] > thr;
+                if (inflow && k == 0)
+                    act |= Uin[(long)i * lp->ny + j] > thr;
+                if (!act)
+                    continue;
+                int q[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    if (q[a] < lo[a])
+                        lo[a] = q[a];
+                    if (q[a] > hi[a])
+                        hi[a] = q[a];
+                }
+            }
+    if (hi[0] < 0)
+        return 0;
+    for (int a = 0; a < 3; ++a) {
+        out[2 * a] = b[2 * a] + lo[a];
+        out[2 * a + 1] = b[2 * a] + hi[a] + 1;
+    }
+    return 1;
+}
+
+/* moving_active_domain.py:196-232 (also the crop of a full-grid state and
+ * embed_state, 235-244, as boxes of the whole grid): dst on box nb holds src
+ * on box ob where they overlap and zero elsewhere. */
+static void rebox(const double *src, const int ob[6], double *dst, const int nb[6])
+{
+    int onx = ob[1] - ob[0], ony = ob[3] - ob[2], onz = ob[5] - ob[4];
+    int nnx = nb[1] - nb[0], nny = nb[3] - nb[2], nnz = nb[5] - nb[4];
+    long on = (long)onx * ony * onz, nn = (long)nnx * nny * nnz;
+    memset(dst, 0, sizeof(double) * 4 * (size_t)nn);
+    int g0[3], g1[3];
+    for (int a = 0; a < 3; ++a) {
+        g0[a] = ob[2 * a] > nb[2 * a] ? ob[2 * a] : nb[2 * a];
+        g1[a] = ob[2 * a + 1] < nb[2 * a + 1] ? ob[2 * a + 1] : nb[2 * a + 1];
+        if (g1[a] <= g0[a])
+            return;
+    }
+    for (int c = 0; c < 4; ++c)
+        for (int gi = g0[0]; gi < g1[0]; ++gi)
+            for (int gj = g0[1]; gj < g1[1]; ++gj) {
+                long so = c * on + ((long)(gi - ob[0]) * ony + (gj - ob[2])) * onz + (g0[2] - ob[4]);
+                long d = c * nn + ((long)(gi - nb[0]) * nny + (gj - nb[2])) * nnz + (g0[2] - nb[4]);
+                memcpy(dst + d, src + so, sizeof(double) * (size_t)(g1[2] - g0[2]));
+            }
+}
+
+/* energy.py:1262-1494 with moving_active_domain.py:35-253. */
+int or_energy_march_moving(const or_problem *p, const or_config *c, int guard,
+                           double truncation_sigma, int vacuum_aware,
+                           const double *U0, double *U_out, double *dose,
+                           double *energies, int cap, int *n_energies,
+                           const double *snap_E, int n_snap, double *snap_out,
+                           int *snap_taken, int max_steps, int32_t *boxes)
+{
+    long n = (long)p->nx * p->ny * p->nz, m = (long)p->nx * p->ny;
+    int full[6] = {0, p->nx, 0, p->ny, 0, p->nz};
+    *n_energies = 0;
+    if (p->n_mat > 64 || p->n_mat < 1)
+        return fail("between 1 and 64 materials are supported");
+    if (guard < 4)
+        return fail("active_guard_cells must be an integer >= 4.");
+    if (!(truncation_sigma > 0.0))
+        return fail("active_truncation_sigma must be positive.");
+    if (mat_check(p))
+        return -1;
+    double energy = p->beam_energy + c->energy_window_sigma * p->energy_spread;
+    double emin = c->energy_min;
+    double tol = exp(-0.5 * (truncation_sigma * truncation_sigma));
+    int ns = n_snap > 0 ? n_snap : 1;
+    double *targets = malloc(sizeof(double) * (size_t)ns);
+    int *tmap = malloc(sizeof(int) * (size_t)ns);
+    double *U = malloc(sizeof(double) * 4 * (size_t)n);
+    double *W = malloc(sizeof(double) * 4 * (size_t)n);
+    double *edep = malloc(sizeof(double) * (size_t)n);
+    int32_t *ids = malloc(sizeof(int32_t) * (size_t)n);
+    double *spatial = malloc(sizeof(double) * (size_t)m);
+    double *Uin = malloc(sizeof(double) * 4 * (size_t)m);
+    int rc = -1;
+    if (!targets || !tmap || !U || !W || !edep || !ids || !spatial || !Uin) {
+        fail("out of memory");
+        goto out;
+    }
+    int nt = 0;
+    for (int i = 0; i < n_snap; ++i) {
+        if (snap_taken)
+            snap_taken[i] = 0;
+        double t = snap_E[i];
+        if (!(emin <= t && t <= energy))
+            continue;
+        int dup = 0;
+        for (int k = 0; k < nt; ++k)
+            dup |= targets[k] == t;
+        if (!dup)
+            targets[nt++] = t;
+    }
+    qsort(targets, (size_t)nt, sizeof(double), cmp_desc);
+    for (int k = 0; k < nt; ++k)
+        for (int i = 0; i < n_snap; ++i)
+            if (snap_E[i] == targets[k]) {
+                tmap[k] = i;
+                break;
+            }
+    double rho[64];
+    for (int q = 0; q < p->n_mat; ++q)
+        rho[q] = p->mats[q].density;
+    int box[6];
+    if (initial_box(p, guard, box))
+        goto out;
+    if (U0)
+        rebox(U0, full, U, box);
+    else
+        memset(U, 0, sizeof(double) * 4 * (size_t)n);
+    memset(dose, 0, sizeof(double) * (size_t)n);
+    int ne = 0, ti = 0, steps = 0;
+    if (ne < cap)
+        energies[ne] = energy;
+    ne++;
+    or_problem lp;
+    while (energy > emin) {
+        if (max_steps > 0 && steps >= max_steps)
+            break;
+        local_problem(p, box, ids, spatial, &lp);
+        int sig[6], next[6];
+        int r = significant_box(&lp, box, U, energy, tol, Uin, sig);
+        if (r < 0)
+            goto out;
+        if (r == 0) {
+            if (initial_box(p, guard, next))
+                goto out;
+        } else {
+            next[0] = sig[0] - guard > 0 ? sig[0] - guard : 0;
+            next[1] = sig[1] + guard < p->nx ? sig[1] + guard : p->nx;
+            next[2] = sig[2] - guard > 0 ? sig[2] - guard : 0;
+            next[3] = sig[3] + guard < p->ny ? sig[3] + guard : p->ny;
+            next[4] = sig[4] - guard > 0 ? sig[4] - guard : 0;
+            next[5] = sig[5] + guard < p->nz ? sig[5] + guard : p->nz;
+        }
+        if (memcmp(next, box, sizeof box) != 0) {
+            rebox(U, box, W, next);
+            double *t = U;
+            U = W;
+            W = t;
+            memcpy(box, next, sizeof box);
+            local_problem(p, box, ids, spatial, &lp);
+        }
+        if (boxes && steps < cap)
+            memcpy(boxes + 6 * (size_t)steps, box, sizeof box);
+        double dE;
+        if (energy_step(&lp, c, U, energy, vacuum_aware ? tol : -1.0, &dE))
+            goto out;
+        if (ti < nt) {
+            double t = targets[ti];
+            if (energy > t && t >= energy - dE)
+                dE = energy - t;
+        }
+        if (energy - emin < dE)
+            dE = energy - emin;
+        if (c->energy_scheme == OR_IF_SSPRK3)
+            r = or_if_ssprk3(&lp, c, U, energy, dE, edep);
+        else if (c->scattering != OR_SCAT_OFF || c->nuclear)
+            r = or_strang_step(&lp, c, U, energy, dE, edep);
+        else
+            r = advance(&lp, c, U, energy, dE, edep);
+        if (r)
+            goto out;
+        for (int i = 0; i < lp.nx; ++i)
+            for (int j = 0; j < lp.ny; ++j)
+                for (int k = 0; k < lp.nz; ++k) {
+                    long l = ((long)i * lp.ny + j) * lp.nz + k;
+                    long g = ((long)(box[0] + i) * p->ny + (box[2] + j)) * p->nz + (box[4] + k);
+                    dose[g] += edep[l] * MEV_TO_JOULE / rho[ids[l]];
+                }
+        energy -= dE;
+        if (ne < cap)
+            energies[ne] = energy;
+        ne++;
+        steps++;
+        if (ti < nt) {
+            double t = targets[ti];
+            if (fabs(energy - t) <= 1e-12) {
+                int slot = tmap[ti];
+                if (snap_out)
+                    rebox(U, box, snap_out + (size_t)slot * 4 * (size_t)n, full);
+                if (snap_taken)
+                    snap_taken[slot] = 1;
+                ti++;
+            }
+        }
+    }
+    if (U_out)
+        rebox(U, box, U_out, full);
+    *n_energies = ne;
+    rc = 0;
+out:
+    free(targets);
+    free(tmap);
+    free(U);
+    free(W);
+    free(edep);
+    free(ids);
+    free(spatial);
+    free(Uin);
     return rc;
 }

@@ -115,6 +115,13 @@ int or_inflow_state(const or_problem *p, double energy, double *U_in);
 /* energy.py:539-660 (vacuum_relative_tolerance=None). */
 int or_compute_energy_step(const or_problem *p, const or_config *c,
                            const double *U, double energy, double *dE_out);
+/* energy.py:539-660 with vacuum_relative_tolerance given (energy.py:459-537):
+ * the directional speeds of cells more than 3 cells (the WENO-Z stencil
+ * reach) along that axis from any cell with phi > tolerance * max phi (the
+ * inflow phi included at z = 0) are dropped. */
+int or_compute_energy_step_vacuum(const or_problem *p, const or_config *c,
+                                  const double *U, double energy,
+                                  double relative_tolerance, double *dE_out);
 /* energy.py:663-722, 725-873, 878-1130, 1194-1260: one step each.  U is
  * replaced by the new state, edep receives the energy deposition [MeV/m^3
  * per proton-fluence unit] of the step. */
@@ -137,6 +144,21 @@ int or_energy_march(const or_problem *p, const or_config *c, double *U,
                     double *dose, double *energies, int cap, int *n_energies,
                     const double *snap_E, int n_snap, double *snap_out,
                     int *snap_taken, int max_steps);
+
+/* energy.py:1262-1494 (moving_active_domain=True) with the box helpers of
+ * moving_active_domain.py:35-253.  The state lives on a box that follows the
+ * cells with phi > exp(-sigma^2/2) * max phi, widened by `guard` cells; the
+ * dose is accumulated on the full grid.  U0 (full grid, 4N) may be NULL (zero
+ * start on the initial box); U_out (4N, may be NULL) receives the final box
+ * state embedded in the full grid; snapshots are full-grid states.  boxes
+ * (may be NULL) receives per step the box {i0, i1, j0, j1, k0, k1} the step
+ * was taken on (capacity cap steps).  Other arguments as or_energy_march. */
+int or_energy_march_moving(const or_problem *p, const or_config *c, int guard,
+                           double truncation_sigma, int vacuum_aware,
+                           const double *U0, double *U_out, double *dose,
+                           double *energies, int cap, int *n_energies,
+                           const double *snap_E, int n_snap, double *snap_out,
+                           int *snap_taken, int max_steps, int32_t *boxes);
 
 #ifdef __cplusplus
 }

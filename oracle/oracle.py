@@ -95,6 +95,10 @@ def lib():
             getattr(L, name).argtypes = [P, Cf, _dp, C.c_double, C.c_double, _dp]
         L.or_energy_march.argtypes = [P, Cf, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int),
                                       _dp, C.c_int, _dp, _ip, C.c_int]
+        L.or_compute_energy_step_vacuum.argtypes = [P, Cf, _dp, C.c_double, C.c_double, _dp]
+        L.or_energy_march_moving.argtypes = [P, Cf, C.c_int, C.c_double, C.c_int, _dp, _dp,
+                                             _dp, _dp, C.c_int, C.POINTER(C.c_int), _dp,
+                                             C.c_int, _dp, _ip, C.c_int, _ip]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_rk3_trace_combination.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
@@ -174,6 +178,7 @@ class OracleProblem:
         self.ids = np.ascontiguousarray(np.asarray(material_map.ids), dtype=np.int32)
         self.spatial = np.ascontiguousarray(np.asarray(inflow.spatial), dtype=np.float64)
         self._keep = []
+        self.materials = list(material_map.materials)
         mats = (or_material * len(material_map.materials))()
         for i, m in enumerate(material_map.materials):
             mats[i], keep = material_struct(m)
@@ -224,6 +229,47 @@ class OracleProblem:
         _check(lib().or_compute_energy_step(C.byref(self.prob), C.byref(self.cfg),
                                             _ptr(U), energy, C.byref(out)))
         return out.value
+
+    def compute_energy_step_vacuum(self, U, energy, relative_tolerance):
+        """compute_energy_step with vacuum_relative_tolerance (energy.py:459-660)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        out = C.c_double()
+        _check(lib().or_compute_energy_step_vacuum(C.byref(self.prob), C.byref(self.cfg),
+                                                   _ptr(U), energy, float(relative_tolerance),
+                                                   C.byref(out)))
+        return out.value
+
+    def march_moving(self, U=None, snapshot_energies=(), max_steps=0, guard=None,
+                     truncation_sigma=None, vacuum_aware=None):
+        """_moving_active_energy_march: returns (U, dose [Gy], energies, snapshots,
+        boxes), boxes[s] = (i0, i1, j0, j1, k0, k1) of step s.  guard, sigma and
+        the vacuum-aware CFL default to the config's numerics."""
+        num = self.config.numerics
+        guard = num.active_guard_cells if guard is None else guard
+        sigma = num.active_truncation_sigma if truncation_sigma is None else truncation_sigma
+        va = num.vacuum_aware_cfl if vacuum_aware is None else vacuum_aware
+        U0 = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        U_out = np.empty((4, *self.shape))
+        dose = np.empty(self.shape)
+        cap = 1 << 16
+        energies = np.empty(cap)
+        boxes = np.zeros((cap, 6), dtype=np.int32)
+        ne = C.c_int(0)
+        snaps = np.ascontiguousarray([float(s) for s in snapshot_energies], dtype=np.float64)
+        ns = len(snaps)
+        snap_out = np.empty((max(ns, 1), 4, *self.shape)) if ns else None
+        taken = np.zeros(max(ns, 1), dtype=np.int32)
+        _check(lib().or_energy_march_moving(
+            C.byref(self.prob), C.byref(self.cfg), int(guard), float(sigma), int(bool(va)),
+            _ptr(U0), _ptr(U_out), _ptr(dose), _ptr(energies), cap, C.byref(ne),
+            _ptr(snaps) if ns else None, ns, _ptr(snap_out), _ptr(taken), int(max_steps),
+            _ptr(boxes)))
+        snapshots = {}
+        for i in range(ns):
+            if taken[i]:
+                snapshots[float(snaps[i])] = snap_out[i].copy()
+        k = min(ne.value, cap)
+        return U_out, dose, energies[:k].copy(), snapshots, boxes[: max(k - 1, 0)].copy()
 
     def step(self, U, energy, dE, kind="if_ssprk3"):
         """One integrator step; returns (U_new, energy_deposition)."""

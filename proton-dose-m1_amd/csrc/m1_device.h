@@ -95,6 +95,11 @@ struct CflArgs {
     const StepCoef* coef;   // uses S0, ed[0], rf
     unsigned long long* out;  // max of the inverse step, as ordered bits
     int wave_mode;
+    // Vacuum-aware CFL (energy.py:459-537): per-cell activity (nullable).  An
+    // axis's speed counts only where an active cell lies within act_radius
+    // cells along that axis.
+    const uint8_t* act;
+    int act_radius;
 };
 
 struct BoundaryArgs {
@@ -449,6 +454,64 @@ struct BatchKernels {
 #define M1_BATCH_TABLE_NAME(a, b, c) batch_kernels_##a##x##b##x##c
 #define M1_DECLARE_BATCH_TABLE(a, b, c) const BatchKernels& M1_BATCH_TABLE_NAME(a, b, c)();
 M1_BATCH_SHAPES(M1_DECLARE_BATCH_TABLE)
+
+// ------------------------------------------------ moving active domain (FP64)
+// A box of the grid: half-open global cell ranges [i0, i1) x [j0, j1) x [k0, k1).
+struct CellBox {
+    int i0, i1, j0, j1, k0, k1;
+};
+
+// Reductions over the box state: the largest phi (order key, 0 = none) and
+// the index range of the significant cells (lo = INT_MAX, hi = -1 = none).
+struct BoxReduce {
+    unsigned long long phi_max;
+    int lo[3];
+    int hi[3];
+};
+
+// Order-preserving 64-bit key of a double (negative values below positive,
+// -0 below +0, NaN of either sign at the ends); 0 is below every key.
+__host__ __device__ inline unsigned long long order_key(double v)
+{
+    unsigned long long b;
+    __builtin_memcpy(&b, &v, sizeof b);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double key_value(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    __builtin_memcpy(&v, &b, sizeof v);
+    return v;
+}
+
+namespace box {
+// dst (4 components on box db) = src (on box sb) where the boxes overlap,
+// zero elsewhere; crop, rebox and embed are this copy between boxes.
+hipError_t launch_copy(const double* src, CellBox sb, double* dst, CellBox db, hipStream_t s);
+// ids and inflow profile of box b, cut from the full grid (ny, nz: full grid);
+// the profile is zero unless the box touches z = 0.
+hipError_t launch_gather(const uint8_t* ids, const double* spatial, int ny, int nz, CellBox b,
+                         uint8_t* box_ids, double* box_spatial, hipStream_t s);
+// dose (full grid) += local (box b), local = 0.
+hipError_t launch_scatter_dose(double* dose, int ny, int nz, CellBox b, double* local,
+                               hipStream_t s);
+// r = {phi_max: 0, lo: INT_MAX, hi: -1}
+hipError_t launch_reset(BoxReduce* r, hipStream_t s);
+// r->phi_max = max(r->phi_max, key(phi)) over n cells
+hipError_t launch_phi_max(const double* phi, long n, BoxReduce* r, hipStream_t s);
+// Significant cells of a box of dims g: phi > tol * scale, and at z = 0
+// (inflow_on) spatial * ed > tol * scale, scale = max(phi_max, phi_in_max if
+// inflow_on); their index range goes to r->lo / r->hi.
+hipError_t launch_significant(const double* phi, GridDims g, const double* spatial, double ed,
+                              int inflow_on, double phi_in_max, double tol, BoxReduce* r,
+                              hipStream_t s);
+// act = phi > tol * scale or (z = 0 and spatial * ed > tol * scale), scale =
+// max(phi_max, phi_in_max); every cell active when scale == 0.
+hipError_t launch_activity(const double* phi, GridDims g, const double* spatial, double ed,
+                           double phi_in_max, double tol, const BoxReduce* r, uint8_t* act,
+                           hipStream_t s);
+}  // namespace box
 
 namespace f32 {
 hipError_t launch_stage(int kind, bool integrating_factor, const StageArgs32& a, hipStream_t s);

@@ -665,6 +665,24 @@ __global__ void __launch_bounds__(CFL_THREADS) cfl_kernel(CflArgs a)
                 eigen(pin, jin, jnin, lo, hi);
                 az = fmax(az, fmax(fabs(lo), fabs(hi)));
             }
+            if (a.act) {
+                // energy.py:624-630: an axis's speed counts only within
+                // act_radius cells (along that axis) of an active cell.
+                const long sx = long(ny) * nz;
+                const long ix = i / sx, iy = (i / nz) % ny, iz = i % nz;
+                bool mx_on = false, my_on = false, mz_on = false;
+                for (int d = -a.act_radius; d <= a.act_radius; ++d) {
+                    if (ix + d >= 0 && ix + d < nx)
+                        mx_on |= a.act[i + d * sx] != 0;
+                    if (iy + d >= 0 && iy + d < ny)
+                        my_on |= a.act[i + d * nz] != 0;
+                    if (iz + d >= 0 && iz + d < nz)
+                        mz_on |= a.act[i + d] != 0;
+                }
+                ax = mx_on ? ax : 0.0;
+                ay = my_on ? ay : 0.0;
+                az = mz_on ? az : 0.0;
+            }
             v = ax / (s * a.g.dx) + ay / (s * a.g.dy) + az / (s * a.g.dz);
         }
         if (!(v == v))

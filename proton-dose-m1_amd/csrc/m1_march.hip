@@ -61,6 +61,20 @@ struct m1_ctx {
     double *U = nullptr, *V1 = nullptr, *V2 = nullptr, *work = nullptr, *dose = nullptr,
            *edep = nullptr, *bnd = nullptr, *spatial = nullptr;
     uint8_t* ids = nullptr;
+    std::vector<double> h_spatial;  // inflow profile (nx, ny), host copy
+    // The cells the FP64 kernels march: the whole grid, or the current box
+    // of the moving active domain (its ids and inflow profile cut out).
+    struct Domain {
+        int nx = 0, ny = 0, nz = 0;
+        long N = 0;
+        const uint8_t* ids = nullptr;
+        const double* spatial = nullptr;
+    } dom;
+    // moving active domain buffers, allocated on first use
+    uint8_t *box_ids = nullptr, *box_act = nullptr;
+    double *box_spatial = nullptr, *box_dose = nullptr;
+    m1::BoxReduce* d_boxred = nullptr;
+    m1::BoxReduce* h_boxred = nullptr;
     // FP32-bulk buffers, allocated on first use.
     float *U32 = nullptr, *V1_32 = nullptr, *V2_32 = nullptr, *work32 = nullptr,
           *dose_hi = nullptr, *dose_lo = nullptr, *edep32 = nullptr, *bnd32 = nullptr;
@@ -97,8 +111,8 @@ void m1_ctx::free_problem()
                     (void*)bnd, (void*)spatial, (void*)ids, (void*)U32, (void*)V1_32,
                     (void*)V2_32, (void*)work32, (void*)dose_hi, (void*)dose_lo, (void*)edep32,
                     (void*)bnd32, (void*)d_compute[0], (void*)d_compute[1], (void*)d_tile_max,
-                    (void*)d_tile_face,
-                    (void*)d_nactive})
+                    (void*)d_tile_face, (void*)d_nactive, (void*)box_ids, (void*)box_act,
+                    (void*)box_spatial, (void*)box_dose})
         if (p)
             (void)hipFree(p);
     U = V1 = V2 = work = dose = edep = bnd = spatial = nullptr;
@@ -107,6 +121,9 @@ void m1_ctx::free_problem()
     d_tile_max = d_tile_face = nullptr;
     d_nactive = nullptr;
     ids = nullptr;
+    box_ids = box_act = nullptr;
+    box_spatial = box_dose = nullptr;
+    dom = Domain{};
     have_problem = false;
 }
 
@@ -116,10 +133,11 @@ m1_ctx::~m1_ctx()
     if (stream)
         (void)hipStreamSynchronize(stream);
     free_problem();
-    for (void* p : {(void*)d_step, (void*)d_cfl, (void*)d_red, (void*)d_step32})
+    for (void* p : {(void*)d_step, (void*)d_cfl, (void*)d_red, (void*)d_step32, (void*)d_boxred})
         if (p)
             (void)hipFree(p);
-    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red, (void*)h_step32, (void*)h_nactive})
+    for (void* p : {(void*)h_step, (void*)h_cfl, (void*)h_red, (void*)h_step32, (void*)h_nactive,
+                    (void*)h_boxred})
         if (p)
             (void)hipHostFree(p);
     prof.release();
@@ -129,6 +147,10 @@ m1_ctx::~m1_ctx()
 
 namespace {
 
+// Reach of the WENO-Z reconstruction (ghost width 3, energy.py:612-619): the
+// dilation radius of the vacuum-aware CFL activity.
+constexpr int STENCIL_REACH = 3;
+
 void sync(m1_ctx* c)
 {
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -137,7 +159,12 @@ void sync(m1_ctx* c)
 }
 
 // ------------------------------------------------------------ numerics
-m1::GridDims dims(const m1_ctx* c) { return {c->nx, c->ny, c->nz, c->dx, c->dy, c->dz}; }
+// Dimensions of the marched domain (the whole grid except inside
+// m1_march_moving, which marches a box of it).
+m1::GridDims dims(const m1_ctx* c)
+{
+    return {c->dom.nx, c->dom.ny, c->dom.nz, c->dx, c->dy, c->dz};
+}
 
 BeamParams beam_params(const m1_ctx* c)
 {
@@ -162,7 +189,7 @@ m1::StageArgs base_args(const m1_ctx* c, const Num& n)
 {
     m1::StageArgs a{};
     a.g = dims(c);
-    a.ids = c->ids;
+    a.ids = c->dom.ids;
     a.coef = c->d_step;
     a.wave_mode = n.wave;
     return a;
@@ -177,7 +204,7 @@ void upload_step(m1_ctx* c)
 
 void run_boundary(m1_ctx* c, const Num& n, bool ifac)
 {
-    m1::BoundaryArgs b{dims(c), c->spatial, c->ids, c->d_step, c->bnd, ifac ? 1 : 0};
+    m1::BoundaryArgs b{dims(c), c->dom.spatial, c->dom.ids, c->d_step, c->bnd, ifac ? 1 : 0};
     Timed t(c->prof, c->stream, K_BOUNDARY);
     hipError_t e = n.precision == M1_FP64 ? m1::exact::launch_boundary(b, c->stream)
                                                  : m1::fast::launch_boundary(b, c->stream);
@@ -186,7 +213,7 @@ void run_boundary(m1_ctx* c, const Num& n, bool ifac)
 
 void run_scale(m1_ctx* c, const Num& n)
 {
-    m1::ScaleArgs s{c->N, c->U, c->ids, c->d_step, n.nuclear ? 1 : 0,
+    m1::ScaleArgs s{c->dom.N, c->U, c->dom.ids, c->d_step, n.nuclear ? 1 : 0,
                     n.scattering != M1_SCATTERING_OFF ? 1 : 0};
     Timed t(c->prof, c->stream, K_SCALE);
     hip_check(m1::fast::launch_scale(s, c->stream), "source half-step kernel");
@@ -197,7 +224,7 @@ void run_scale(m1_ctx* c, const Num& n)
 void rk3(m1_ctx* c, const Num& n, bool ifac, double* dose, double* edep)
 {
     run_boundary(c, n, ifac);
-    const long m2 = long(c->nx) * c->ny;
+    const long m2 = long(c->dom.nx) * c->dom.ny;
     m1::StageArgs a = base_args(c, n);
     a.U = c->U;
     a.V1 = c->V1;
@@ -285,8 +312,9 @@ void step(m1_ctx* c, const Num& n, int kind, double energy, double dE, double* d
     }
 }
 
-// compute_energy_step (energy.py:539-660) of the device state at `energy`.
-double cfl_step(m1_ctx* c, const Num& n, double energy)
+// compute_energy_step (energy.py:539-660) of the device state at `energy`;
+// `act` (nullable) is the cell activity of the vacuum-aware CFL.
+double cfl_step(m1_ctx* c, const Num& n, double energy, const uint8_t* act = nullptr)
 {
     StepCoef& s = *c->h_cfl;
     m1::fill_cfl(c->mats, beam_params(c), energy, s);
@@ -295,7 +323,8 @@ double cfl_step(m1_ctx* c, const Num& n, double energy)
               "CFL coefficient upload");
     hip_check(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream),
               "CFL reset");
-    m1::CflArgs a{dims(c), c->U, c->ids, c->spatial, c->d_cfl, c->d_red, n.wave};
+    m1::CflArgs a{dims(c), c->U,   c->dom.ids, c->dom.spatial, c->d_cfl,
+                  c->d_red, n.wave, act,       act ? STENCIL_REACH : 0};
     {
         Timed t(c->prof, c->stream, K_CFL);
         hipError_t e = n.precision == M1_FP64 ? m1::exact::launch_cfl(a, c->stream)
@@ -582,6 +611,143 @@ void widen_state(m1_ctx* c, double* out)
               "state widening");
 }
 
+// ------------------------------------------------- moving active domain
+// _moving_active_energy_march (energy.py:1262-1494) with the box helpers of
+// moving_active_domain.py:35-253.  The state lives on a box of the grid that
+// follows the significant cells; the box is made the marched domain (its ids
+// and inflow profile cut out by a kernel), so the step kernels run on it
+// unchanged, and the box's dose is added into the full-grid dose after every
+// step.
+using m1::CellBox;
+
+CellBox full_box(const m1_ctx* c) { return {0, c->nx, 0, c->ny, 0, c->nz}; }
+
+long cells(const CellBox& b) { return long(b.i1 - b.i0) * (b.j1 - b.j0) * (b.k1 - b.k0); }
+
+bool same_box(const CellBox& a, const CellBox& b)
+{
+    return a.i0 == b.i0 && a.i1 == b.i1 && a.j0 == b.j0 && a.j1 == b.j1 && a.k0 == b.k0 &&
+           a.k1 == b.k1;
+}
+
+// initial_box (moving_active_domain.py:35-60): the beam's transverse support
+// widened by `guard`, z in [0, 1 + guard).
+CellBox initial_box(const m1_ctx* c, int guard)
+{
+    int ix0 = -1, ix1 = -1, iy0 = c->ny, iy1 = -1;
+    for (int i = 0; i < c->nx; ++i)
+        for (int j = 0; j < c->ny; ++j)
+            if (c->h_spatial[size_t(i) * c->ny + j] > 0.0) {
+                if (ix0 < 0)
+                    ix0 = i;
+                ix1 = i;
+                iy0 = std::min(iy0, j);
+                iy1 = std::max(iy1, j);
+            }
+    if (ix0 < 0)
+        throw ValueError{"Beam inflow has empty transverse support."};
+    return {std::max(0, ix0 - guard), std::min(c->nx, ix1 + 1 + guard),
+            std::max(0, iy0 - guard), std::min(c->ny, iy1 + 1 + guard),
+            0,                        std::min(c->nz, 1 + guard)};
+}
+
+// Largest inflow phi of box b at spectrum value ed: max over the box's
+// columns of profile * ed (boundary.py:82-114), zero once the box has left
+// z = 0 (make_local_problem, moving_active_domain.py:84-92).
+double inflow_phi_max(const m1_ctx* c, const CellBox& b, double ed)
+{
+    if (b.k0 != 0)
+        return 0.0;
+    double mx = c->h_spatial[size_t(b.i0) * c->ny + b.j0] * ed;
+    for (int i = b.i0; i < b.i1; ++i)
+        for (int j = b.j0; j < b.j1; ++j) {
+            const double v = c->h_spatial[size_t(i) * c->ny + j] * ed;
+            if (v > mx)
+                mx = v;
+        }
+    return mx;
+}
+
+void ensure_box_buffers(m1_ctx* c)
+{
+    if (c->box_ids)
+        return;
+    dev_alloc(&c->box_ids, size_t(c->N), "hipMalloc box ids");
+    dev_alloc(&c->box_act, size_t(c->N), "hipMalloc box activity");
+    dev_alloc(&c->box_spatial, size_t(c->nx) * c->ny, "hipMalloc box inflow");
+    dev_alloc(&c->box_dose, size_t(c->N), "hipMalloc box dose");
+    if (!c->d_boxred)
+        dev_alloc(&c->d_boxred, 1, "hipMalloc box reduction");
+    if (!c->h_boxred)
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_boxred), sizeof(m1::BoxReduce)),
+                  "hipHostMalloc");
+    hip_check(hipMemsetAsync(c->box_dose, 0, size_t(c->N) * sizeof(double), c->stream),
+              "box dose reset");
+}
+
+// Make box b the marched domain.
+void set_box_domain(m1_ctx* c, const CellBox& b)
+{
+    hip_check(m1::box::launch_gather(c->ids, c->spatial, c->ny, c->nz, b, c->box_ids,
+                                     c->box_spatial, c->stream),
+              "box gather kernel");
+    c->dom = {b.i1 - b.i0, b.j1 - b.j0, b.k1 - b.k0, cells(b), c->box_ids, c->box_spatial};
+}
+
+void set_full_domain(m1_ctx* c) { c->dom = {c->nx, c->ny, c->nz, c->N, c->ids, c->spatial}; }
+
+// The state U on box `from` becomes the state on box `to` (zero outside the
+// overlap; moving_active_domain.py:196-244).
+void move_state(m1_ctx* c, const CellBox& from, const CellBox& to)
+{
+    hip_check(m1::box::launch_copy(c->U, from, c->work, to, c->stream), "box copy kernel");
+    std::swap(c->U, c->work);
+}
+
+// desired_box (moving_active_domain.py:154-193) of the state on box b at
+// `energy`: significant_box (107-151) widened by `guard`, or initial_box
+// when no cell is significant.
+CellBox desired_box(m1_ctx* c, const CellBox& b, int guard, double tol, double ed)
+{
+    const m1::GridDims g = dims(c);
+    const int inflow_on = b.k0 == 0;
+    const double phi_in_max = inflow_phi_max(c, b, ed);
+    hip_check(m1::box::launch_reset(c->d_boxred, c->stream), "box reduction reset");
+    hip_check(m1::box::launch_phi_max(c->U, c->dom.N, c->d_boxred, c->stream), "phi max kernel");
+    hip_check(m1::box::launch_significant(c->U, g, c->dom.spatial, ed, inflow_on, phi_in_max, tol,
+                                          c->d_boxred, c->stream),
+              "significant box kernel");
+    hip_check(hipMemcpyAsync(c->h_boxred, c->d_boxred, sizeof(m1::BoxReduce),
+                             hipMemcpyDeviceToHost, c->stream),
+              "box reduction read-back");
+    sync(c);
+    const m1::BoxReduce& r = *c->h_boxred;
+    double scale = m1::key_value(r.phi_max);
+    if (inflow_on && phi_in_max > scale)
+        scale = phi_in_max;
+    if (scale == 0.0 || r.hi[0] < 0)
+        return initial_box(c, guard);
+    return {std::max(0, b.i0 + r.lo[0] - guard), std::min(c->nx, b.i0 + r.hi[0] + 1 + guard),
+            std::max(0, b.j0 + r.lo[1] - guard), std::min(c->ny, b.j0 + r.hi[1] + 1 + guard),
+            std::max(0, b.k0 + r.lo[2] - guard), std::min(c->nz, b.k0 + r.hi[2] + 1 + guard)};
+}
+
+// compute_energy_step with vacuum_relative_tolerance = tol (energy.py:493-660)
+// of the state on the marched domain; `b` is that domain's box of the grid.
+double cfl_step_vacuum(m1_ctx* c, const Num& n, double energy, const CellBox& b, double tol)
+{
+    if (n.wave != 0)
+        return cfl_step(c, n, energy);  // the masks apply to the exact M1 speeds only
+    const double ed = m1::energy_density(energy, c->beam_energy, c->energy_spread);
+    hip_check(m1::box::launch_reset(c->d_boxred, c->stream), "box reduction reset");
+    hip_check(m1::box::launch_phi_max(c->U, c->dom.N, c->d_boxred, c->stream), "phi max kernel");
+    hip_check(m1::box::launch_activity(c->U, dims(c), c->dom.spatial, ed,
+                                       inflow_phi_max(c, b, ed), tol, c->d_boxred, c->box_act,
+                                       c->stream),
+              "activity kernel");
+    return cfl_step(c, n, energy, c->box_act);
+}
+
 }  // namespace
 
 extern "C" {
@@ -703,6 +869,8 @@ int m1_set_problem(m1_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double dx, d
         hip_check(hipMemsetAsync(ctx->dose, 0, size_t(N) * sizeof(double), ctx->stream),
                   "dose reset");
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->h_spatial.assign(inflow_spatial, inflow_spatial + m2);
+        ctx->dom = {nx, ny, nz, N, ctx->ids, ctx->spatial};
         ctx->have_problem = true;
     });
 }
@@ -853,6 +1021,143 @@ int m1_march(m1_ctx* ctx, const m1_numerics* num, double* energies, int32_t ener
         sync(ctx);
         if (n_energies)
             *n_energies = ne;
+    });
+}
+
+int m1_march_moving(m1_ctx* ctx, const m1_numerics* num, int32_t guard_cells,
+                    double truncation_sigma, double* energies, int32_t energies_cap,
+                    int32_t* n_energies, const double* snapshot_energies, int32_t n_snapshots,
+                    double* snapshots, int32_t* snapshot_taken, int32_t* boxes)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const Num n = read_numerics(num);
+        if (m1::is_f32(n))
+            throw NotImplemented{"moving_active_domain is marched in FP64 and FP64_FAST only."};
+        if (guard_cells < 4)
+            throw ValueError{"active_guard_cells must be an integer >= 4."};
+        if (!(truncation_sigma > 0.0) || !std::isfinite(truncation_sigma))
+            throw ValueError{"active_truncation_sigma must be positive."};
+        if (n_snapshots < 0 || (n_snapshots > 0 && !snapshot_energies))
+            throw ValueError{"invalid snapshot request"};
+        m1_ctx* c = ctx;
+        double energy = c->beam_energy + n.window * c->energy_spread;
+        const double emin = n.emin;
+        const double tol = std::exp(-0.5 * (truncation_sigma * truncation_sigma));
+        std::vector<double> targets;
+        for (int i = 0; i < n_snapshots; ++i) {
+            if (snapshot_taken)
+                snapshot_taken[i] = 0;
+            const double t = snapshot_energies[i];
+            if (emin <= t && t <= energy &&
+                std::find(targets.begin(), targets.end(), t) == targets.end())
+                targets.push_back(t);
+        }
+        std::sort(targets.begin(), targets.end(), std::greater<double>());
+        CellBox box = initial_box(c, guard_cells);
+        const CellBox full = full_box(c);
+        ensure_box_buffers(c);
+        hip_check(hipMemsetAsync(c->dose, 0, size_t(c->N) * sizeof(double), c->stream),
+                  "dose reset");
+        // the device state (m1_set_state) is cut to the initial box
+        move_state(c, full, box);
+        set_box_domain(c, box);
+        int32_t ne = 0;
+        auto record = [&](double e) {
+            if (energies && ne < energies_cap)
+                energies[ne] = e;
+            ++ne;
+        };
+        const int kind = n.scheme == M1_IF_SSPRK3 ? M1_STEP_IF_SSPRK3
+                         : has_sources(n)         ? M1_STEP_STRANG
+                         : n.scheme == M1_FORWARD_EULER ? M1_STEP_FORWARD_EULER
+                                                        : M1_STEP_SSPRK3;
+        try {
+            record(energy);
+            size_t ti = 0;
+            int32_t steps = 0;
+            while (energy > emin) {
+                const double ed = m1::energy_density(energy, c->beam_energy, c->energy_spread);
+                const CellBox next = desired_box(c, box, guard_cells, tol, ed);
+                if (!same_box(next, box)) {
+                    move_state(c, box, next);
+                    box = next;
+                    set_box_domain(c, box);
+                }
+                if (boxes && steps < energies_cap) {
+                    const int32_t b6[6] = {box.i0, box.i1, box.j0, box.j1, box.k0, box.k1};
+                    std::memcpy(boxes + 6 * size_t(steps), b6, sizeof b6);
+                }
+                double dE = n.vacuum_aware_cfl ? cfl_step_vacuum(c, n, energy, box, tol)
+                                               : cfl_step(c, n, energy);
+                if (ti < targets.size()) {
+                    const double t = targets[ti];
+                    if (energy > t && t >= energy - dE)
+                        dE = energy - t;
+                }
+                if (energy - emin < dE)
+                    dE = energy - emin;
+                step(c, n, kind, energy, dE, c->box_dose, nullptr);
+                hip_check(m1::box::launch_scatter_dose(c->dose, c->ny, c->nz, box, c->box_dose,
+                                                       c->stream),
+                          "dose scatter kernel");
+                energy -= dE;
+                record(energy);
+                ++steps;
+                if (ti < targets.size()) {
+                    const double t = targets[ti];
+                    if (std::fabs(energy - t) <= 1e-12) {
+                        if (snapshots) {
+                            int slot = 0;
+                            for (int i = 0; i < n_snapshots; ++i)
+                                if (snapshot_energies[i] == t) {
+                                    slot = i;
+                                    break;
+                                }
+                            hip_check(m1::box::launch_copy(c->U, box, c->V1, full, c->stream),
+                                      "box copy kernel");
+                            hip_check(hipMemcpyAsync(snapshots + size_t(slot) * 4 * size_t(c->N),
+                                                     c->V1, 4 * size_t(c->N) * sizeof(double),
+                                                     hipMemcpyDeviceToHost, c->stream),
+                                      "snapshot download");
+                            if (snapshot_taken)
+                                snapshot_taken[slot] = 1;
+                        }
+                        ++ti;
+                    }
+                }
+            }
+        } catch (...) {
+            // leave a full-grid state and domain behind
+            set_full_domain(c);
+            (void)m1::box::launch_copy(c->U, box, c->work, full, c->stream);
+            std::swap(c->U, c->work);
+            (void)hipStreamSynchronize(c->stream);
+            throw;
+        }
+        // embed_state (moving_active_domain.py:235-244): the final state on the grid
+        move_state(c, box, full);
+        set_full_domain(c);
+        sync(c);
+        if (n_energies)
+            *n_energies = ne;
+    });
+}
+
+int m1_compute_energy_step_vacuum(m1_ctx* ctx, const m1_numerics* num, double energy,
+                                  double relative_tolerance, double* dE)
+{
+    return guarded(ctx, [&] {
+        require_problem(ctx);
+        const Num n = read_numerics(num);
+        if (m1::is_f32(n))
+            throw NotImplemented{"the vacuum-aware CFL is computed in FP64 and FP64_FAST only."};
+        if (!(relative_tolerance >= 0.0))
+            throw ValueError{"relative tolerance must be non-negative"};
+        ensure_box_buffers(ctx);
+        const double d = cfl_step_vacuum(ctx, n, energy, full_box(ctx), relative_tolerance);
+        if (dE)
+            *dE = d;
     });
 }
 

@@ -16,7 +16,7 @@ import numpy as np
 from .enums import CoefficientInterpolation, EnergyScheme, Precision, Scattering, WaveSpeeds
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libm1march.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 M1_OK = 0
 M1_ERR_VALUE = -1
@@ -81,6 +81,11 @@ SIGNATURES = [
                            C.POINTER(C.c_int32), _dp, C.c_int32, _dp, _ip]),
     ("m1_spatial_operator", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_double, _dp, _dp]),
     ("m1_compute_energy_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_double, _dp]),
+    ("m1_march_moving", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_int32, C.c_double,
+                                  _dp, C.c_int32, C.POINTER(C.c_int32), _dp, C.c_int32, _dp,
+                                  _ip, _ip]),
+    ("m1_compute_energy_step_vacuum", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_double,
+                                                C.c_double, _dp]),
     ("m1_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_int32, C.c_double,
                           C.c_double, _dp]),
     ("m1_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
@@ -318,6 +323,36 @@ class Context:
             if taken[i]:
                 snapshots[float(snaps[i])] = snap_out[i]
         return energies[: min(ne.value, cap)].copy(), snapshots
+
+    def march_moving(self, num: m1_numerics, guard_cells: int, truncation_sigma: float,
+                     snapshot_energies=()):
+        """Moving active-domain march: (energies, {E: U}, boxes) with boxes[s] =
+        (i0, i1, j0, j1, k0, k1) of step s; dose and the embedded final state
+        stay on the device."""
+        cap = 1 << 20
+        energies = np.empty(cap)
+        boxes = np.zeros((cap, 6), dtype=np.int32)
+        ne = C.c_int32(0)
+        snaps = np.ascontiguousarray([float(s) for s in snapshot_energies], dtype=np.float64)
+        ns = len(snaps)
+        snap_out = np.empty((ns, 4, *self.shape)) if ns else None
+        taken = np.zeros(max(ns, 1), dtype=np.int32)
+        self._check(self.lib.m1_march_moving(
+            self.h, C.byref(num), int(guard_cells), float(truncation_sigma), _ptr(energies), cap,
+            C.byref(ne), _ptr(snaps) if ns else None, ns, _ptr(snap_out), _ptr(taken),
+            _ptr(boxes)))
+        snapshots = {}
+        for i in range(ns):
+            if taken[i]:
+                snapshots[float(snaps[i])] = snap_out[i]
+        k = min(ne.value, cap)
+        return energies[:k].copy(), snapshots, boxes[: max(k - 1, 0)].copy()
+
+    def compute_energy_step_vacuum(self, num, energy, relative_tolerance):
+        out = C.c_double()
+        self._check(self.lib.m1_compute_energy_step_vacuum(
+            self.h, C.byref(num), float(energy), float(relative_tolerance), C.byref(out)))
+        return out.value
 
     def spatial_operator(self, num, energy, boundary_state=None):
         b = None

@@ -65,10 +65,8 @@ def energy_march(
 ):
     """March the M1 system from high to low kinetic energy (energy.py:1497-1644)."""
     if config.numerics.moving_active_domain:
-        raise NotImplementedError(
-            "moving_active_domain is not available on the MI355X march; "
-            "the full-domain march gives the static-domain result."
-        )
+        return _moving_active_energy_march(U, material_map, grid, inflow, config, xp,
+                                           snapshot_energies)
     if U is None:
         raise ValueError(
             "A full-grid initial state is required when moving_active_domain is disabled."
@@ -84,13 +82,44 @@ def energy_march(
     return U_out, dose, energies, snapshots
 
 
+def _moving_active_energy_march(U, material_map, grid, inflow, config, xp=np,
+                                snapshot_energies=()):
+    """March on a box that follows the protons (energy.py:1262-1494).
+
+    The state lives on the bounding box of the cells with
+    phi > exp(-active_truncation_sigma^2 / 2) * max phi, widened by
+    active_guard_cells and re-found before every step; the dose is kept on the
+    whole grid.  U may be None (zero start, as Solver passes it) or a
+    full-grid state, which is cut to the initial box.  The final state is the
+    box state embedded in the grid, or None when output.store_final_state is
+    False.  The march runs in libm1march.so (m1_march_moving).
+    """
+    num_cfg = config.numerics
+    ctx = _bind(None if U is None else _host(U), material_map, grid, inflow, config)
+    num = _native.numerics_struct(config)
+    energies, snapshots, _ = ctx.march_moving(num, num_cfg.active_guard_cells,
+                                              num_cfg.active_truncation_sigma,
+                                              snapshot_energies)
+    dose = ctx.get_dose()
+    store = getattr(getattr(config, "output", None), "store_final_state", True)
+    U_out = ctx.get_state() if store else None
+    if xp is not np:
+        dose = xp.asarray(dose)
+        U_out = None if U_out is None else xp.asarray(U_out)
+        snapshots = {k: xp.asarray(v) for k, v in snapshots.items()}
+    return U_out, dose, energies, snapshots
+
+
 def compute_energy_step(U, energy, material_map, grid, inflow, config, xp=np, *,
                         vacuum_relative_tolerance=None):
-    """Stable energy step from the spatial CFL condition (energy.py:539-660)."""
-    if vacuum_relative_tolerance is not None:
-        raise NotImplementedError("vacuum-aware CFL belongs to the moving active domain.")
+    """Stable energy step from the spatial CFL condition (energy.py:539-660);
+    with vacuum_relative_tolerance the vacuum-aware directional masks of
+    energy.py:459-537 apply."""
     ctx = _bind(_host(U), material_map, grid, inflow, config)
-    return ctx.compute_energy_step(_native.numerics_struct(config), energy)
+    num = _native.numerics_struct(config)
+    if vacuum_relative_tolerance is not None:
+        return ctx.compute_energy_step_vacuum(num, energy, vacuum_relative_tolerance)
+    return ctx.compute_energy_step(num, energy)
 
 
 def _one_step(kind, U, energy, dE, material_map, grid, inflow, config, xp):

@@ -36,7 +36,9 @@ class Solver:
         snapshots = {}
         runtime = None
         if cfg.solve_deterministic:
-            U = xp.zeros((4, *grid.shape), dtype=float)
+            # The moving active domain starts from U = None (solver.py:90-100).
+            U = None if cfg.numerics.moving_active_domain else xp.zeros((4, *grid.shape),
+                                                                        dtype=float)
             start = time.perf_counter()
             U, dose, energies, snapshots = energy_march(
                 U, material_map, grid, inflow, cfg, xp, snapshot_energies=snapshot_energies

@@ -41,10 +41,14 @@ def config_from_fixture(c: dict) -> SolverConfig:
                   sigma=tuple(b["sigma"]), center=tuple(b["center"]),
                   angular_spread=b["angular_spread"], cutoff_sigma=b["cutoff_sigma"]),
         physics=Physics(scattering=Scattering[p["scattering"]], nuclear=p["nuclear"],
-                        coefficient_interpolation=CoefficientInterpolation[p["coefficient_interpolation"]]),
+                        coefficient_interpolation=CoefficientInterpolation[
+                            p.get("coefficient_interpolation", "LINEAR")]),
         numerics=Numerics(spatial_scheme=SpatialScheme[n["spatial_scheme"]],
                           energy_scheme=EnergyScheme[n["energy_scheme"]],
                           wave_speeds=WaveSpeeds[n["wave_speeds"]], cfl=n["cfl"],
                           energy_min=n["energy_min"], energy_window_sigma=n["energy_window_sigma"],
-                          max_energy_step=n["max_energy_step"]),
+                          max_energy_step=n["max_energy_step"],
+                          **{k: n[k] for k in ("moving_active_domain", "active_guard_cells",
+                                               "active_truncation_sigma", "vacuum_aware_cfl")
+                             if k in n}),
     )

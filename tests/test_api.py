@@ -113,13 +113,15 @@ def test_cpu_backend_is_not_this_package():
         resolve_backend(Backend.CPU)
 
 
-def test_energy_march_argument_errors():
-    from proton_solver.numerics.energy import energy_march
+def test_energy_march_argument_errors(monkeypatch):
+    from proton_solver.numerics import energy as E
 
+    # moving_active_domain=True goes to the moving-box march, U = None allowed
     cfg = SolverConfig(geometry=geo(), beam=Beam(energy=100),
                        numerics=Numerics(moving_active_domain=True))
-    with pytest.raises(NotImplementedError, match="moving_active_domain"):
-        energy_march(None, None, None, None, cfg)
+    monkeypatch.setattr(E, "_moving_active_energy_march", lambda *a, **k: "moving")
+    assert E.energy_march(None, None, None, None, cfg) == "moving"
+    energy_march = E.energy_march
     cfg = SolverConfig(geometry=geo(), beam=Beam(energy=100))
     with pytest.raises(ValueError, match="full-grid initial state"):
         energy_march(None, None, None, None, cfg)
