@@ -57,6 +57,20 @@ class Result:
         return None
 
     @property
+    def reference_label(self):
+        """Name of the selected reference (result.py:119-140)."""
+        if (self.dose is None and self.analytical_dose is not None
+                and (self.bfp_mc_dose is not None or self.topas_dose is not None)):
+            return "Analytical"
+        if self.bfp_mc_dose is not None:
+            return "BFP Monte Carlo"
+        if self.topas_dose is not None:
+            return "TOPAS"
+        if self.analytical_dose is not None:
+            return "Analytical"
+        return None
+
+    @property
     def maximum_dose(self):
         import numpy as np
 

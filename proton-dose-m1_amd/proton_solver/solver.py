@@ -53,7 +53,12 @@ class Solver:
                     f"Reference {ref.name} is not computed by the MI355X march package.",
                     stacklevel=2,
                 )
-        return Result(
+        result = Result(
             phi=phi, J=J, dose=dose, grid=grid, energies=energies, snapshots=snapshots,
             runtime=runtime, backend=self.backend.backend,
         )
+        if cfg.output.save_results:
+            from .persistence import save_result
+
+            result.run_directory = save_result(result, cfg)
+        return result

@@ -60,6 +60,9 @@ def main():
                 "peak_search_from_mm": met["range"].get("peak_search_from_mm"),
                 "central_axis_peak_depth_mm": met["central_axis_bragg_peak"]["deterministic_depth_mm"],
                 "runtime_seconds": meta.get("runtime_seconds"),
+                # the stored configuration as written and its hash (persistence)
+                "config_json": cfg,
+                "config_hash": meta.get("config_hash"),
             })
     OUT.write_text(json.dumps({"cases": cases}, indent=1, sort_keys=True) + "\n")
     print(f"wrote {len(cases)} cases to {OUT}")
