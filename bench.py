@@ -143,10 +143,11 @@ def roofline_of(stats, stage_bytes, cells_per_launch, traffic_per_cell=None):
     }
 
 
-def batch_config(tile_threshold=1e-10):
-    from proton_solver.batch import batch_workload_config
+def batch_config(tile_threshold=None):
+    from proton_solver.batch import BATCH_TILE_THRESHOLD, batch_workload_config
 
-    return batch_workload_config(tile_threshold)
+    return batch_workload_config(BATCH_TILE_THRESHOLD if tile_threshold is None
+                                 else tile_threshold)
 
 
 def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_beamlet):
@@ -364,7 +365,7 @@ def run_batch(args):
             "box_cells_total": int(box_cells.sum()),
             "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
                                          "max": int(steps.max())},
-            "tile_threshold": args.tile_threshold,
+            "tile_threshold": cfg.device.tile_threshold,
         },
         "cell_steps_per_s": cell_steps / elapsed,
         "cell_steps_per_s_evaluated": cell_steps * eval_frac / elapsed,
@@ -396,7 +397,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--beamlets", type=int, default=256)
     ap.add_argument("--fp64-sample", type=int, default=3)
-    ap.add_argument("--tile-threshold", type=float, default=1e-10)
+    ap.add_argument("--tile-threshold", type=float, default=None,
+                    help="tile threshold relative to the peak inflow fluence "
+                         "(default: proton_solver.batch.BATCH_TILE_THRESHOLD)")
     ap.add_argument("--skip-network", default="auto", choices=["auto", "on", "off"],
                     help="tile-skip network in the batched march (auto: when its weights exist)")
     ap.add_argument("--no-network-reference", action="store_true",

@@ -86,7 +86,7 @@ def _march(handle, num, model, options):
     return dose, steps, tiles, judged, rejections, back, per
 
 
-def test_recorded_samples(samples):
+def test_recorded_samples(samples, cfg):
     x, y = samples
     assert len(y) > 1000 and x.shape == (len(y), 11)
     assert np.isfinite(x).all()
@@ -94,7 +94,9 @@ def test_recorded_samples(samples):
     # candidates have nothing above the threshold within 4 cells; with the
     # whole-phantom CFL's fine steps nearly all stay empty over 8 steps
     assert y.mean() > 0.5
-    assert (x[:, 0] >= -3.0).all() and (x[:, 0] <= -1.0).all()   # own max <= threshold 1e-10
+    # own max at or below the tile threshold (feature: log10(max phi) / 10)
+    lg_thr = np.log10(cfg.device.tile_threshold) / 10
+    assert (x[:, 0] >= -3.0).all() and (x[:, 0] <= lg_thr + 1e-6).all()
     assert (x[:, 2] >= 0).all() and (x[:, 2] <= 1).all()
     assert (x[:, 3] > 0.2).all() and (x[:, 3] < 2.0).all()       # g/cm^3
     assert (x[:, 4] > 0).all() and (x[:, 4] <= 1).all()
