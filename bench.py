@@ -59,8 +59,13 @@ STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 69}
 # on a kernel of known traffic (profiles/stage_kernels_batch_pmc_traffic.md);
 # shapes without a measurement report traffic null.
 PMC_BYTES_PER_CELL_FP32 = {
-    (16, 16, 16): {"stage0": 58.6, "stage1": 84.6, "stage2": 105.5},
-    (8, 16, 16): {"stage0": 59.2, "stage1": 86.5, "stage2": 106.1},
+    (8, 16, 16): {"stage0": 66.2, "stage1": 93.3, "stage2": 114.0},
+}
+# Fraction of the VALU issue slots the batched stage kernels use, per tile
+# shape (rocprofv3 SQ_INSTS_VALU x 4 cycles / (kernel time x 2.4 GHz x 1024
+# SIMDs), profiles/stage_kernels_batch_valu.md): the bound of these kernels.
+VALU_ISSUE_FRAC_FP32 = {
+    (8, 16, 16): {"stage0": 0.89, "stage1": 0.78, "stage2": 0.74},
 }
 
 
@@ -119,9 +124,10 @@ def cpu_baseline(cfg, n_steps_full, sample_steps):
     }
 
 
-def roofline_of(stats, stage_bytes, cells_per_launch, traffic_per_cell=None):
+def roofline_of(stats, stage_bytes, cells_per_launch, traffic_per_cell=None, valu_frac=None):
     """HBM roofline of the stage kernel with the largest total time; traffic
-    is the measured HBM bytes per launch when per-cell figures are given."""
+    is the measured HBM bytes per launch when per-cell figures are given, and
+    valu_issue_frac_profiled the kernel's profiled VALU issue fraction."""
     best = None
     for name, per_cell in stage_bytes.items():
         launches, total_ms = stats.get(name, (0, 0.0))
@@ -140,6 +146,7 @@ def roofline_of(stats, stage_bytes, cells_per_launch, traffic_per_cell=None):
         "traffic": (round(traffic_per_cell[name] * cells) if traffic_per_cell else None),
         "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
         "algorithmic_bytes_per_cell": per_cell, "cells_per_launch": round(cells),
+        "valu_issue_frac_profiled": (valu_frac or {}).get(name),
     }
 
 
@@ -278,7 +285,8 @@ def run_batch(args):
         return ev * float(tx * ty * tz) * fill / launches
 
     roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch,
-                       PMC_BYTES_PER_CELL_FP32.get(tuple(handle.tile_shape)))
+                       PMC_BYTES_PER_CELL_FP32.get(tuple(handle.tile_shape)),
+                       VALU_ISSUE_FRAC_FP32.get(tuple(handle.tile_shape)))
     if args.profile_calibration:
         # a kernel of known traffic for scaling the HBM counters: the beamlet
         # dose read-out reads a float pair and writes a double per box cell
