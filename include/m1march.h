@@ -205,6 +205,23 @@ int m1_march_moving(m1_ctx *ctx, const m1_numerics *num, int32_t guard_cells,
 int m1_compute_energy_step_vacuum(m1_ctx *ctx, const m1_numerics *num, double energy,
                                   double relative_tolerance, double *dE);
 
+/* Gamma index on the device: no reference counterpart (the reference's
+ * metrics.py:542-606 calls pymedphys).  Counts the points (x, y, z voxel
+ * triplets, n_points of them) whose global gamma^2 is <= 1: for each point p,
+ * over the search offsets (offsets: n_offsets voxel-unit triplets, sorted so
+ * that dist_terms = (length / distance)^2 is non-decreasing), the evaluated
+ * dose is trilinearly interpolated at p + offset (clipped to the grid) and
+ * gamma^2 = min ((D - D_ref(p)) / dose_tolerance)^2 + dist_term.  dose and
+ * reference are host (nx, ny, nz) arrays; device memory is allocated and
+ * freed inside the call.  The arithmetic is that of
+ * proton_solver.metrics.gamma_pass_rate, so the count equals the host
+ * search's.  m1_gamma_last_error gives the message of a failed call. */
+int m1_gamma_pass_count(int32_t device, const double *dose, const double *reference,
+                        int32_t nx, int32_t ny, int32_t nz, const int32_t *points,
+                        int32_t n_points, const double *offsets, const double *dist_terms,
+                        int32_t n_offsets, double dose_tolerance, int64_t *n_pass);
+const char *m1_gamma_last_error(void);
+
 /* spatial_operator (spatial.py:104-252, active_domain=False) of the current
  * device state at `energy`: rhs = -div F_HLL, (4,nx,ny,nz).  boundary_state
  * is the z-min inflow state (4,nx,ny) or NULL for inflow.state(energy). */

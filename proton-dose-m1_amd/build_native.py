@@ -61,6 +61,7 @@ def _units():
         (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=off", *waves]),
         (CSRC / "m1_march.hip", BUILD / "m1_march.o", ["-ffp-contract=off", *waves]),
         (CSRC / "m1_box_kernels.hip", BUILD / "m1_box_kernels.o", ["-ffp-contract=off"]),
+        (CSRC / "m1_gamma.hip", BUILD / "m1_gamma.o", ["-ffp-contract=off"]),
         *[(CSRC / "m1_batch_kernels.hip", BUILD / f"m1_batch_kernels_{x}x{y}x{z}.o",
            ["-ffp-contract=off", *waves, f"-DM1_TILE_TX={x}", f"-DM1_TILE_TY={y}",
             f"-DM1_BATCH_NS=batch_{x}x{y}x{z}", f"-DM1_BATCH_TABLE=batch_kernels_{x}x{y}x{z}"])

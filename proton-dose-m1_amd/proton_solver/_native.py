@@ -86,6 +86,10 @@ SIGNATURES = [
                                   _ip, _ip]),
     ("m1_compute_energy_step_vacuum", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_double,
                                                 C.c_double, _dp]),
+    ("m1_gamma_pass_count", C.c_int, [C.c_int32, _dp, _dp, C.c_int32, C.c_int32, C.c_int32, _ip,
+                                      C.c_int32, _dp, _dp, C.c_int32, C.c_double,
+                                      C.POINTER(C.c_int64)]),
+    ("m1_gamma_last_error", C.c_char_p, []),
     ("m1_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_int32, C.c_double,
                           C.c_double, _dp]),
     ("m1_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
@@ -185,6 +189,26 @@ FP64 = 0
 FP32_BULK = 1
 FP64_FAST = 2
 FP32_BULK_F64_CLOSURE = 3
+
+
+def gamma_pass_count(device, dose, reference, points, offsets, dist_terms, dose_tolerance):
+    """Number of points with gamma <= 1 (m1_gamma_pass_count; the arguments
+    are those proton_solver.metrics.gamma_pass_rate prepares)."""
+    lib = load_library()
+    dose = np.ascontiguousarray(dose, dtype=np.float64)
+    reference = np.ascontiguousarray(reference, dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.int32)
+    offs = np.ascontiguousarray(offsets, dtype=np.float64)
+    dt = np.ascontiguousarray(dist_terms, dtype=np.float64)
+    nx, ny, nz = dose.shape
+    n = C.c_int64(0)
+    rc = lib.m1_gamma_pass_count(int(device), _ptr(dose), _ptr(reference), nx, ny, nz, _ptr(pts),
+                                 len(pts), _ptr(offs), _ptr(dt), len(dt), float(dose_tolerance),
+                                 C.byref(n))
+    if rc != M1_OK:
+        msg = lib.m1_gamma_last_error().decode()
+        raise (ValueError if rc == M1_ERR_VALUE else MarchError)(msg)
+    return int(n.value)
 
 
 def numerics_struct(config, precision_code: int | None = None) -> m1_numerics:

@@ -90,18 +90,11 @@ def range_metrics(dose, z, z_from=None):
     return r80, r20 - r80
 
 
-def gamma_pass_rate(dose, reference, spacing_m, dose_percent=1.0, distance_mm=1.0,
-                    lower_percent_cutoff=10.0, interp_fraction=10, max_gamma=1.1):
-    """Global 3-D gamma pass rate in percent.
-
-    Evaluated points are reference voxels at or above ``lower_percent_cutoff``
-    of the reference maximum.  For each, the evaluated dose is searched on a
-    grid of offsets with step ``distance/interp_fraction`` inside a sphere of
-    radius ``max_gamma * distance``; the evaluated dose between voxel centres is
-    trilinearly interpolated.  gamma = min over offsets of
-    sqrt((dD / (dose_percent% * max_ref))^2 + (r / distance)^2); a point passes
-    when gamma <= 1.
-    """
+def _gamma_setup(dose, reference, spacing_m, dose_percent, distance_mm, lower_percent_cutoff,
+                 interp_fraction, max_gamma):
+    """Evaluated points, search offsets (in voxels, sorted by length), their
+    distance terms and the dose tolerance of gamma_pass_rate; None when no
+    point is evaluated."""
     dose = np.asarray(dose, dtype=float)
     ref = np.asarray(reference, dtype=float)
     if dose.shape != ref.shape:
@@ -126,17 +119,47 @@ def gamma_pass_rate(dose, reference, spacing_m, dose_percent=1.0, distance_mm=1.
                 if r <= reach + 1e-15:
                     offs.append((a * step, b * step, c * step, r))
     offs = np.asarray(offs)
-    offs = offs[np.argsort(offs[:, 3])]
-    sp = np.asarray(spacing_m, dtype=float)
+    offs = offs[np.argsort(offs[:, 3], kind="stable")]
+    scaled = offs[:, :3] / np.asarray(spacing_m, dtype=float)
+    dist_terms = np.array([(r / dist) ** 2 for r in offs[:, 3]])
+    return dose, ref, pts, scaled, dist_terms, dd
+
+
+def gamma_pass_rate(dose, reference, spacing_m, dose_percent=1.0, distance_mm=1.0,
+                    lower_percent_cutoff=10.0, interp_fraction=10, max_gamma=1.1,
+                    device=None):
+    """Global 3-D gamma pass rate in percent.
+
+    Evaluated points are reference voxels at or above ``lower_percent_cutoff``
+    of the reference maximum.  For each, the evaluated dose is searched on a
+    grid of offsets with step ``distance/interp_fraction`` inside a sphere of
+    radius ``max_gamma * distance``; the evaluated dose between voxel centres is
+    trilinearly interpolated.  gamma = min over offsets of
+    sqrt((dD / (dose_percent% * max_ref))^2 + (r / distance)^2); a point passes
+    when gamma <= 1.
+
+    ``device`` (a HIP device index) evaluates the points on the GPU
+    (m1_gamma_pass_count in libm1march.so), bit-identical to the host search.
+    """
+    setup = _gamma_setup(dose, reference, spacing_m, dose_percent, distance_mm,
+                         lower_percent_cutoff, interp_fraction, max_gamma)
+    if setup is None:
+        return None
+    dose, ref, pts, scaled, dist_terms, dd = setup
+    if device is not None:
+        from . import _native
+
+        return 100.0 * _native.gamma_pass_count(device, dose, ref, pts, scaled, dist_terms,
+                                                dd) / len(pts)
     shape = np.asarray(dose.shape)
     gamma2 = np.full(len(pts), np.inf)
     ref_vals = ref[tuple(pts.T)]
-    for ox, oy, oz, r in offs:
-        dist_term = (r / dist) ** 2
+    for k in range(len(scaled)):
+        dist_term = dist_terms[k]
         alive = gamma2 > dist_term
         if not np.any(alive):
             break
-        p = pts[alive].astype(float) + np.array([ox, oy, oz]) / sp
+        p = pts[alive].astype(float) + scaled[k]
         p = np.clip(p, 0.0, shape - 1.0)
         i0 = np.floor(p).astype(int)
         i0 = np.minimum(i0, shape - 2).clip(min=0)
