@@ -311,6 +311,15 @@ int m1_batch_tile_stats(m1_batch *batch, int64_t *evaluated, int64_t *full);
 int m1_batch_reset_stats(m1_batch *batch);
 /* Host time of the per-step physics scalars since the last reset, ms. */
 int m1_batch_host_ms(m1_batch *batch, double *setup_ms);
+
+/* CFL reduction of the batch (no reference counterpart).  By default the
+ * plain CFL evaluates the cells of the live tiles (those that may hold a
+ * non-zero state) from the compacted tile list and takes the tiles at the
+ * zero state by their materials (their isotropic speeds) and z-min inflow
+ * columns; enable = 1 evaluates every cell of every tile of the boxes
+ * instead.  Both give the same maximum bit for bit. */
+int m1_batch_set_cfl_all_tiles(m1_batch *batch, int32_t enable);
+
 /* Stage kernels on a persistent grid (the workgroups resident on the
  * device) that take tiles from eight per-XCD queues, stealing across XCDs
  * when their own is empty (enable 1), or one workgroup per tile (0, the

@@ -195,6 +195,97 @@ __device__ void cfl_tile_vacuum_aware(const BatchCflArgs& a, const BoxDesc& d,
 // CFL maximum per box over all its cells.  Tiles outside the live set hold
 // the exact zero state: their cells take the vacuum value of their material
 // (and of the inflow on z-min cells) without reading the state.
+// Every cell of one tile that may hold a non-zero state.
+__device__ __forceinline__ void cfl_live_tile(const BatchCflArgs& a, const BoxDesc& d,
+                                              const TileCoord& tc, int b, double* red)
+{
+    const StepCoef& c = a.coef[b];
+    const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
+    const long N = long(d.nx) * d.ny * d.nz;
+    const double eps = c.eig_eps;
+    const double pin_scale = c.ed[0] * c.inv_alpha, rf = c.rf;
+    const uint8_t* ids = a.ids + d.cell_off;
+    const double* spatial = a.spatial + d.col_off;
+    const int tid = threadIdx.x;
+    const int x0 = tc.tx * TX, y0 = tc.ty * TY, z0 = tc.tz * TZ;
+    const int x1 = min(x0 + TX, d.nx);
+    double mx = 0.0;
+    bool bad = false;
+    const float* U = a.U + 4 * d.cell_off;
+    const int z = z0 + tid % TZ, y = y0 + tid / TZ;
+    if (z < d.nz && y < d.ny) {
+        // unrolled over the tile's x planes so that the loads of all of a
+        // lane's cells are in flight together
+#pragma unroll
+        for (int k = 0; k < TX; ++k) {
+            const int x = x0 + k;
+            if (x < x1) {
+                const long i = (long(x) * d.ny + y) * d.nz + z;
+                const bool zmin = z == 0;
+                const double pin = zmin ? spatial[i / d.nz] * pin_scale : 0.0;
+                const double v =
+                    cfl_cell(U, i, N, c.S0[ids[i]], g, a.wave_mode, eps, zmin, pin, rf);
+                if (!(v == v))
+                    bad = true;
+                else
+                    mx = fmax(mx, v);
+            }
+        }
+    }
+    block_max_to(mx, bad, red, a.out + b);
+}
+
+// Live-list form of the plain CFL: the workgroups walk the compacted list of
+// live tiles.
+__global__ void __launch_bounds__(NT) cfl_list_kernel(BatchCflArgs a)
+{
+    __shared__ double red[NT / 64];
+    const int n = *a.count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int t = a.list[i];
+        const int b = a.tile_box[t];
+        if (a.step[b].active) {
+            const BoxDesc d = a.box[b];
+            cfl_live_tile(a, d, tile_coord(d, b, t), b, red);
+        }
+        __syncthreads();  // red is reused by the next tile
+    }
+}
+
+// Live-list form, tiles outside the live set (exact zero state), one thread
+// per tile: the tile's materials go to its box's mask (the host adds their
+// vacuum values), and a z-min tile's columns add the inflow speeds.
+__global__ void __launch_bounds__(256) cfl_vacuum_tiles_kernel(BatchCflArgs a)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.n_tiles)
+        return;
+    const int b = a.tile_box[t];
+    if (!a.step[b].active || a.live[t])
+        return;
+    atomicOr(a.box_mats + b, (unsigned long long)a.tile_mats[t]);
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    if (tc.tz != 0)
+        return;
+    const StepCoef& c = a.coef[b];
+    const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
+    const double eps = c.eig_eps;
+    const double pin_scale = c.ed[0] * c.inv_alpha, rf = c.rf;
+    const uint8_t* ids = a.ids + d.cell_off;
+    const double* spatial = a.spatial + d.col_off;
+    const int x0 = tc.tx * TX, y0 = tc.ty * TY;
+    const int x1 = min(x0 + TX, d.nx), y1 = min(y0 + TY, d.ny);
+    double mx = 0.0;
+    for (int x = x0; x < x1; ++x)
+        for (int y = y0; y < y1; ++y) {
+            const long col = long(x) * d.ny + y;
+            const double pin = spatial[col] * pin_scale;
+            mx = fmax(mx, cfl_vacuum(c.S0[ids[col * d.nz]], g, a.wave_mode, eps, true, pin, rf));
+        }
+    atomicMax(a.out + b, (unsigned long long)__double_as_longlong(mx));
+}
+
 __global__ void __launch_bounds__(NT) cfl_kernel(BatchCflArgs a)
 {
     __shared__ double red[NT / 64];
@@ -211,34 +302,21 @@ __global__ void __launch_bounds__(NT) cfl_kernel(BatchCflArgs a)
     }
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
+    if (!a.live || a.live[t]) {
+        cfl_live_tile(a, d, tc, b, red);
+        return;
+    }
     const StepCoef& c = a.coef[b];
     const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
-    const long N = long(d.nx) * d.ny * d.nz;
     const double eps = c.eig_eps;
     const double pin_scale = c.ed[0] * c.inv_alpha, rf = c.rf;
     const uint8_t* ids = a.ids + d.cell_off;
     const double* spatial = a.spatial + d.col_off;
     const int tid = threadIdx.x;
-    const int x0 = tc.tx * TX, y0 = tc.ty * TY, z0 = tc.tz * TZ;
-    const int x1 = min(x0 + TX, d.nx);
+    const int x0 = tc.tx * TX, y0 = tc.ty * TY;
     double mx = 0.0;
-    bool bad = false;
-    if (!a.live || a.live[t]) {
-        const float* U = a.U + 4 * d.cell_off;
-        const int z = z0 + tid % TZ, y = y0 + tid / TZ;
-        if (z < d.nz && y < d.ny) {
-            for (int x = x0; x < x1; ++x) {
-                const long i = (long(x) * d.ny + y) * d.nz + z;
-                const bool zmin = z == 0;
-                const double pin = zmin ? spatial[i / d.nz] * pin_scale : 0.0;
-                const double v = cfl_cell(U, i, N, c.S0[ids[i]], g, a.wave_mode, eps, zmin, pin, rf);
-                if (!(v == v))
-                    bad = true;
-                else
-                    mx = fmax(mx, v);
-            }
-        }
-    } else {
+    const bool bad = false;
+    {
         const unsigned mats = a.tile_mats[t];
         if (tid < MAXMAT && (mats >> tid & 1u))
             mx = cfl_vacuum(c.S0[tid], g, a.wave_mode, eps, false, 0.0, rf);
@@ -644,6 +722,22 @@ hipError_t launch_cfl(const BatchCflArgs& a, hipStream_t s)
 {
     if (a.n_tiles <= 0)
         return hipSuccess;
+    if (a.list && a.live && !a.vacuum_aware) {
+        hipLaunchKernelGGL(cfl_vacuum_tiles_kernel, dim3(unsigned((a.n_tiles + 255) / 256)),
+                           dim3(256), 0, s, a);
+        // a grid of resident workgroups walking the live list
+        static int grid = 0;
+        if (grid == 0) {
+            int dev = 0, cus = 0, per = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, cfl_list_kernel, NT, 0);
+            grid = (cus > 0 ? cus : 256) * (per > 0 ? per : 4);
+        }
+        hipLaunchKernelGGL(cfl_list_kernel, dim3(unsigned(min(grid, a.n_tiles))), dim3(NT), 0,
+                           s, a);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(cfl_kernel, dim3(unsigned(a.n_tiles)), dim3(NT), 0, s, a);
     return hipGetLastError();
 }

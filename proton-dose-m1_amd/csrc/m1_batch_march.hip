@@ -74,6 +74,7 @@ struct m1_batch {
     int* d_list = nullptr;
     int* d_count = nullptr;  // LIST_HEADER ints: length, stage queues
     bool persistent = false; // stage kernels on a fixed grid with XCD queues
+    bool cfl_all_tiles = false;  // CFL over every tile of the boxes (else the live list)
     int persistent_grid = 0; // resident workgroups of a stage kernel
     int cus = 0;             // compute units of the device
     bool use_graphs = false; // capture plain steps as HIP graphs
@@ -84,6 +85,7 @@ struct m1_batch {
     StepCoef* d_cfl = nullptr;
     StepCoef32* d_coef32 = nullptr;
     unsigned long long* d_red = nullptr;
+    unsigned long long* d_boxmats = nullptr;  // per box: materials of its vacuum tiles
     double* d_phantom = nullptr;
     double* d_box_dose = nullptr;
 
@@ -92,6 +94,7 @@ struct m1_batch {
     StepCoef32* h_coef32 = nullptr;
     BoxStep* h_step = nullptr;
     unsigned long long* h_red = nullptr;
+    unsigned long long* h_boxmats = nullptr;
     int* h_count = nullptr;
 
     float* d_tile_rho = nullptr;     // mean density per tile
@@ -177,11 +180,11 @@ void m1_batch::free_problem()
     void* dev[] = {d_box,  d_step,   d_tile_box, d_tile_mats, U,         V1,       V2,
                    dose_hi, dose_lo, bnd,        d_tile_max,  d_tile_face, ids,       spatial,  d_flags[0],
                    d_flags[1], d_list, d_count,  d_coef,      d_cfl,     d_coef32, d_red,
-                   d_phantom, d_box_dose};
+                   d_boxmats, d_phantom, d_box_dose};
     for (void* p : dev)
         if (p)
             (void)hipFree(p);
-    void* host[] = {h_coef, h_cfl, h_coef32, h_step, h_red, h_count};
+    void* host[] = {h_coef, h_cfl, h_coef32, h_step, h_red, h_boxmats, h_count};
     for (void* p : host)
         if (p)
             (void)hipHostFree(p);
@@ -196,12 +199,12 @@ void m1_batch::free_problem()
     d_list = d_count = nullptr;
     d_coef = d_cfl = nullptr;
     d_coef32 = nullptr;
-    d_red = nullptr;
+    d_red = d_boxmats = nullptr;
     d_phantom = d_box_dose = nullptr;
     h_coef = h_cfl = nullptr;
     h_coef32 = nullptr;
     h_step = nullptr;
-    h_red = nullptr;
+    h_red = h_boxmats = nullptr;
     h_count = nullptr;
     have_problem = false;
     marched = false;
@@ -464,6 +467,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         dev_alloc(&c->d_cfl, B, "hipMalloc batch CFL coefficients");
         dev_alloc(&c->d_coef32, B, "hipMalloc batch coefficients (f32)");
         dev_alloc(&c->d_red, B, "hipMalloc batch CFL maxima");
+        dev_alloc(&c->d_boxmats, B, "hipMalloc batch CFL materials");
         dev_alloc(&c->d_box_dose, size_t(max_cells), "hipMalloc beamlet dose");
         dev_alloc(&c->d_tile_rho, T, "hipMalloc tile density");
         dev_alloc(&c->d_tile_sums, 2 * T, "hipMalloc tile sums");
@@ -479,6 +483,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         host_alloc(&c->h_coef32, B, "hipHostMalloc");
         host_alloc(&c->h_step, B, "hipHostMalloc");
         host_alloc(&c->h_red, B, "hipHostMalloc");
+        host_alloc(&c->h_boxmats, B, "hipHostMalloc");
         host_alloc(&c->h_count, 1, "hipHostMalloc");
         hip_check(hipMemcpy(c->d_box, boxes.data(), B * sizeof(BoxDesc), hipMemcpyHostToDevice),
                   "box upload");
@@ -748,15 +753,23 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                      hipMemcpyHostToDevice, s),
                       "CFL coefficient upload");
             hip_check(c->K->clear(c->d_red, B, s), "CFL reset");
+            hip_check(c->K->clear(c->d_boxmats, B, s), "CFL reset");
             {
+                // the live list is the compaction of flags[k] (tail)
+                const bool by_list = !c->cfl_all_tiles;
                 BatchCflArgs ca{c->d_box,  c->d_step,      c->d_tile_box, int(T), c->dx,
                                 c->dy,     c->dz,          c->U,          c->ids, c->spatial,
                                 c->d_flags[k], c->d_tile_mats, c->d_cfl, c->d_red, n.wave,
-                                vacuum_aware ? 1 : 0, thr};
+                                vacuum_aware ? 1 : 0, thr,
+                                by_list ? c->d_list : nullptr, c->d_count, c->d_boxmats};
                 Timed t(c->prof, s, K_CFL);
                 hip_check(c->K->cfl(ca, s), "CFL kernel (batch)");
             }
             hip_check(hipMemcpyAsync(c->h_red, c->d_red, size_t(B) * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, s),
+                      "CFL read-back");
+            hip_check(hipMemcpyAsync(c->h_boxmats, c->d_boxmats,
+                                     size_t(B) * sizeof(unsigned long long),
                                      hipMemcpyDeviceToHost, s),
                       "CFL read-back");
             hip_check(hipMemcpyAsync(c->h_count, c->d_count, sizeof(int), hipMemcpyDeviceToHost,
@@ -874,12 +887,18 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     continue;
                 double mx;
                 std::memcpy(&mx, &c->h_red[b], sizeof mx);
-                if (!vacuum_aware && !n.batch_cfl_box_only) {
-                    // the phantom outside the box, at the zero state in the
-                    // whole-phantom march: isotropic speeds (wave_speeds.py:37-50)
-                    const double iso = n.wave == M1_WAVE_UNIT ? 1.0 : 0.57735026918962576;
+                if (!vacuum_aware) {
+                    // cells at the zero state -- the box's tiles outside the
+                    // live set (materials from the CFL kernel) and, in the
+                    // whole-phantom march, the phantom outside the box:
+                    // isotropic speeds (wave_speeds.py:37-50)
+                    // (the float isotropic speed of the kernels' closure)
+                    const double iso = n.wave == M1_WAVE_UNIT ? 1.0 : double(0.57735026918962576f);
+                    const uint64_t zero_mats =
+                        c->h_boxmats[b] |
+                        (n.batch_cfl_box_only ? 0u : c->outside_mats[size_t(b)]);
                     for (int m = 0; m < int(nm); ++m)
-                        if (c->outside_mats[size_t(b)] >> m & 1u) {
+                        if (zero_mats >> m & 1u) {
                             const double s0 = c->h_cfl[b].S0[m];
                             const double v = iso / (s0 * c->dx) + iso / (s0 * c->dy) +
                                              iso / (s0 * c->dz);
@@ -1174,6 +1193,14 @@ int m1_batch_graph_info(m1_batch* c, int64_t* graph_steps, const char** capture_
             *graph_steps = c->graph_steps;
         if (capture_error)
             *capture_error = c->graph_error.c_str();
+    });
+}
+
+int m1_batch_set_cfl_all_tiles(m1_batch* c, int32_t enable)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->cfl_all_tiles = enable != 0;
     });
 }
 

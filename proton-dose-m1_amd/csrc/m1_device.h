@@ -301,6 +301,13 @@ struct BatchCflArgs {
     int wave_mode;
     int vacuum_aware;          // directional masks from active cells (EXACT speeds only)
     float act_threshold;       // active: normalised phi above this
+    // Live-list form (plain CFL; null list: one workgroup per tile of every
+    // box): the live tiles from the compacted list (count entries), and per
+    // box the OR of tile_mats over its tiles outside the live set, whose
+    // vacuum values the host folds in.
+    const int* list;
+    const int* count;
+    unsigned long long* box_mats;
 };
 
 struct BatchBoundaryArgs {

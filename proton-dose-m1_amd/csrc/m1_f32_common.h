@@ -486,25 +486,27 @@ __device__ __forceinline__ float zmin_ghost(int k, double uin, double u0, double
 __device__ __forceinline__ double amax(double lo, double hi) { return fmax(fabs(lo), fabs(hi)); }
 
 // Inverse CFL step of one cell of the scaled float state (energy.py:539-660):
-// sum over axes of max|lambda| / (S d_axis).  A z-min cell also counts the
-// inflow state (p_in = scaled inflow phi, J = (0, 0, rf p_in)) on the z axis.
+// sum over axes of max|lambda| / (S d_axis).  The speeds of the cell's state
+// come from the float closure of the stage kernels (face_state, eigen_f: the
+// state is already projected, so face_state only forms the closure); a z-min
+// cell also counts the inflow state (p_in = scaled inflow phi,
+// J = (0, 0, rf p_in)) on the z axis, in double.
 __device__ __forceinline__ double cfl_cell(const float* U, long i, long N, double s,
                                            const GridDims& g, int wave_mode, double eps,
                                            bool zmin, double p_in, double rf)
 {
     if (wave_mode == 1)
         return 1.0 / (s * g.dx) + 1.0 / (s * g.dy) + 1.0 / (s * g.dz);
-    const double phi = U[i];
-    const double J[3] = {U[i + N], U[i + 2 * N], U[i + 3 * N]};
-    const double jn = sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
-    Closure c = closure(phi, J, jn);
-    double lo, hi, ax, ay, az;
-    eigen<0>(phi, c, eps, lo, hi);
-    ax = amax(lo, hi);
-    eigen<1>(phi, c, eps, lo, hi);
-    ay = amax(lo, hi);
-    eigen<2>(phi, c, eps, lo, hi);
-    az = amax(lo, hi);
+    const FaceF f = face_state(U[i], U[i + N], U[i + 2 * N], U[i + 3 * N]);
+    const float epsf = float(eps);
+    float l, h;
+    eigen_f<0>(f, epsf, l, h);
+    const double ax = fmaxf(fabsf(l), fabsf(h));
+    eigen_f<1>(f, epsf, l, h);
+    const double ay = fmaxf(fabsf(l), fabsf(h));
+    eigen_f<2>(f, epsf, l, h);
+    double az = fmaxf(fabsf(l), fabsf(h));
+    double lo, hi;
     if (zmin) {
         const double Jin[3] = {0.0, 0.0, rf * p_in};
         const Closure ci = closure(p_in, Jin, fabs(Jin[2]));
@@ -556,7 +558,7 @@ __device__ __forceinline__ double cfl_cell_masked(const float* U, long i, long N
 __device__ __forceinline__ double cfl_vacuum(double s, const GridDims& g, int wave_mode, double eps,
                                              bool zmin, double p_in, double rf)
 {
-    const double iso = 0.57735026918962576;
+    const double iso = 0.57735026918962576f;  // eigen_f's isotropic speed, in float
     if (wave_mode == 1)
         return 1.0 / (s * g.dx) + 1.0 / (s * g.dy) + 1.0 / (s * g.dz);
     double az = iso;

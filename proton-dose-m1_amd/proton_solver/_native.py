@@ -117,6 +117,7 @@ SIGNATURES = [
                                       C.POINTER(C.c_int64)]),
     ("m1_batch_reset_stats", C.c_int, [C.c_void_p]),
     ("m1_batch_host_ms", C.c_int, [C.c_void_p, _dp]),
+    ("m1_batch_set_cfl_all_tiles", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_set_persistent", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_tile_shape_count", C.c_int32, []),
     ("m1_batch_tile_shape", C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -562,6 +563,11 @@ class Batch:
         before set_problem."""
         self._check(self.lib.m1_batch_set_tile_shape(self.h, int(index)))
         self.tile_shape = batch_tile_shapes()[int(index)]
+
+    def set_cfl_all_tiles(self, enable: bool):
+        """Evaluate the CFL over every tile of the boxes (bit-identical to the
+        default live-list form, slower)."""
+        self._check(self.lib.m1_batch_set_cfl_all_tiles(self.h, int(bool(enable))))
 
     def set_persistent(self, enable: bool):
         """Persistent stage kernels with per-XCD tile queues (True) or one

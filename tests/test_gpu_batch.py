@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from proton_solver import (
-    DeviceOptions, EnergyScheme, Material, Numerics, Physics, Precision, Scattering,
+    DeviceOptions, EnergyScheme, Material, Numerics, Physics, Precision, Scattering, WaveSpeeds,
 )
 from proton_solver import _native, metrics
 from proton_solver.batch import (
@@ -214,6 +214,29 @@ def test_persistent_launch_is_bitwise_the_per_tile_launch():
     d1, s1 = h.march(num)
     h.set_persistent(False)
     say("persistent launch: steps", s1.tolist(), "bitwise", bool(np.array_equal(d0, d1)))
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1)
+
+
+@pytest.mark.parametrize("threshold,batch_cfl,wave", [
+    (1e-10, "phantom", WaveSpeeds.EXACT), (1e-6, "phantom", WaveSpeeds.EXACT),
+    (1e-6, "box", WaveSpeeds.EXACT), (1e-6, "phantom", WaveSpeeds.UNIT)])
+def test_cfl_over_the_live_list_is_bitwise_the_cfl_over_every_tile(threshold, batch_cfl, wave):
+    """The CFL reduction over the live tiles, with the zero-state tiles taken
+    by their materials and inflow columns, gives the maximum of every cell
+    of every tile bit for bit: same energy grid, same dose."""
+    ph, beamlets, boxes = small_workload()
+    cfg = _Cfg(tile_threshold=threshold, batch_cfl=batch_cfl)
+    num = _native.numerics_struct(cfg)
+    num.wave_speeds = 0 if wave is WaveSpeeds.EXACT else 1
+    h, _ = prepare_batch(ph, beamlets, cfg, boxes)
+    d0, s0 = h.march(num)
+    h.set_cfl_all_tiles(True)
+    d1, s1 = h.march(num)
+    h.set_cfl_all_tiles(False)
+    ev, full = h.tile_stats()
+    say("CFL forms:", threshold, batch_cfl, wave.name, "steps", s0.tolist(),
+        "bitwise", bool(np.array_equal(d0, d1)))
     assert np.array_equal(s0, s1)
     assert np.array_equal(d0, d1)
 
