@@ -253,8 +253,8 @@ __global__ void __launch_bounds__(NT) cfl_list_kernel(BatchCflArgs a)
 }
 
 // Live-list form, tiles outside the live set (exact zero state), one thread
-// per tile: the tile's materials go to its box's mask (the host adds their
-// vacuum values), and a z-min tile's columns add the inflow speeds.
+// per tile: the tile's materials go to its box's mask, whose vacuum values
+// the host adds (an atomic only when the mask still lacks a bit).
 __global__ void __launch_bounds__(256) cfl_vacuum_tiles_kernel(BatchCflArgs a)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
@@ -263,27 +263,34 @@ __global__ void __launch_bounds__(256) cfl_vacuum_tiles_kernel(BatchCflArgs a)
     const int b = a.tile_box[t];
     if (!a.step[b].active || a.live[t])
         return;
-    atomicOr(a.box_mats + b, (unsigned long long)a.tile_mats[t]);
-    const BoxDesc d = a.box[b];
-    const TileCoord tc = tile_coord(d, b, t);
-    if (tc.tz != 0)
+    const unsigned long long m = a.tile_mats[t];
+    if ((a.box_mats[b] & m) != m)
+        atomicOr(a.box_mats + b, m);
+}
+
+// Live-list form, z-min columns of the tiles outside the live set, one
+// thread per column: the vacuum cell with the inflow speeds on z.
+__global__ void __launch_bounds__(256) cfl_inflow_columns_kernel(BatchCflArgs a)
+{
+    __shared__ double red[256 / 64];
+    const int b = blockIdx.y;
+    if (!a.step[b].active)
         return;
-    const StepCoef& c = a.coef[b];
-    const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
-    const double eps = c.eig_eps;
-    const double pin_scale = c.ed[0] * c.inv_alpha, rf = c.rf;
-    const uint8_t* ids = a.ids + d.cell_off;
-    const double* spatial = a.spatial + d.col_off;
-    const int x0 = tc.tx * TX, y0 = tc.ty * TY;
-    const int x1 = min(x0 + TX, d.nx), y1 = min(y0 + TY, d.ny);
+    const BoxDesc d = a.box[b];
+    const long col = long(blockIdx.x) * 256 + threadIdx.x;
     double mx = 0.0;
-    for (int x = x0; x < x1; ++x)
-        for (int y = y0; y < y1; ++y) {
-            const long col = long(x) * d.ny + y;
-            const double pin = spatial[col] * pin_scale;
-            mx = fmax(mx, cfl_vacuum(c.S0[ids[col * d.nz]], g, a.wave_mode, eps, true, pin, rf));
+    if (col < long(d.nx) * d.ny) {
+        const int x = int(col / d.ny), y = int(col % d.ny);
+        const int t = d.tile_off + ((x / TX) * d.nty + y / TY) * d.ntz;
+        if (!a.live[t]) {
+            const StepCoef& c = a.coef[b];
+            const GridDims g{d.nx, d.ny, d.nz, a.dx, a.dy, a.dz};
+            const double pin = a.spatial[d.col_off + col] * (c.ed[0] * c.inv_alpha);
+            mx = cfl_vacuum(c.S0[a.ids[d.cell_off + col * d.nz]], g, a.wave_mode, c.eig_eps, true,
+                            pin, c.rf);
         }
-    atomicMax(a.out + b, (unsigned long long)__double_as_longlong(mx));
+    }
+    block_max_to(mx, false, red, a.out + b);
 }
 
 __global__ void __launch_bounds__(NT) cfl_kernel(BatchCflArgs a)
@@ -725,6 +732,10 @@ hipError_t launch_cfl(const BatchCflArgs& a, hipStream_t s)
     if (a.list && a.live && !a.vacuum_aware) {
         hipLaunchKernelGGL(cfl_vacuum_tiles_kernel, dim3(unsigned((a.n_tiles + 255) / 256)),
                            dim3(256), 0, s, a);
+        if (a.n_boxes > 0 && a.max_columns > 0)
+            hipLaunchKernelGGL(cfl_inflow_columns_kernel,
+                               dim3(unsigned((a.max_columns + 255) / 256), unsigned(a.n_boxes)),
+                               dim3(256), 0, s, a);
         // a grid of resident workgroups walking the live list
         static int grid = 0;
         if (grid == 0) {

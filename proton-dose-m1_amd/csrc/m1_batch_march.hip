@@ -761,7 +761,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                 c->dy,     c->dz,          c->U,          c->ids, c->spatial,
                                 c->d_flags[k], c->d_tile_mats, c->d_cfl, c->d_red, n.wave,
                                 vacuum_aware ? 1 : 0, thr,
-                                by_list ? c->d_list : nullptr, c->d_count, c->d_boxmats};
+                                by_list ? c->d_list : nullptr, c->d_count, c->d_boxmats,
+                                int(B), c->max_cols};
                 Timed t(c->prof, s, K_CFL);
                 hip_check(c->K->cfl(ca, s), "CFL kernel (batch)");
             }

@@ -308,6 +308,8 @@ struct BatchCflArgs {
     const int* list;
     const int* count;
     unsigned long long* box_mats;
+    int n_boxes;
+    int max_columns;           // largest nx * ny of the boxes
 };
 
 struct BatchBoundaryArgs {
