@@ -78,8 +78,10 @@ __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
                a.wave_mode,
                tc.tx,
                tc.ty,
-               tc.tz};
-    stage_tile<KIND, IFAC, D>(v);
+               tc.tz,
+               a.win ? a.win + off4 : nullptr,
+               a.wout ? a.wout + off4 : nullptr};
+    stage_tile<KIND, IFAC, D, true>(v);
 }
 
 // One workgroup per list entry, or (a.queue set) a persistent grid pulling
@@ -652,6 +654,10 @@ __global__ void __launch_bounds__(NT) zero_kernel(BatchZeroArgs a)
             a.U[off4 + i + c * N] = 0.f;
             a.V1[off4 + i + c * N] = 0.f;
             a.V2[off4 + i + c * N] = 0.f;
+            if (a.W1) {
+                a.W1[off4 + i + c * N] = 0.f;
+                a.W2[off4 + i + c * N] = 0.f;
+            }
         }
     }
 }

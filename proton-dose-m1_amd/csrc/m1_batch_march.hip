@@ -66,6 +66,8 @@ struct m1_batch {
     BoxStep* d_step = nullptr;
     int* d_tile_box = nullptr;
     uint8_t* d_tile_mats = nullptr;
+    // W1, W2: the stencil inputs of stages 1 and 2, formed by stages 0 and 1
+    float *W1 = nullptr, *W2 = nullptr;
     float *U = nullptr, *V1 = nullptr, *V2 = nullptr, *dose_hi = nullptr, *dose_lo = nullptr,
           *bnd = nullptr, *d_tile_max = nullptr, *d_tile_face = nullptr;
     uint8_t* ids = nullptr;
@@ -177,7 +179,7 @@ void m1_batch::free_problem()
             (void)hipFree(p);
     d_tile_rho = d_tile_sums = d_wmax = d_feat = nullptr;
     d_pred = d_ring = d_cand = nullptr;
-    void* dev[] = {d_box,  d_step,   d_tile_box, d_tile_mats, U,         V1,       V2,
+    void* dev[] = {d_box,  d_step,   d_tile_box, d_tile_mats, U,         V1,       V2, W1, W2,
                    dose_hi, dose_lo, bnd,        d_tile_max,  d_tile_face, ids,       spatial,  d_flags[0],
                    d_flags[1], d_list, d_count,  d_coef,      d_cfl,     d_coef32, d_red,
                    d_boxmats, d_phantom, d_box_dose};
@@ -192,7 +194,7 @@ void m1_batch::free_problem()
     d_step = nullptr;
     d_tile_box = nullptr;
     d_tile_mats = nullptr;
-    U = V1 = V2 = dose_hi = dose_lo = bnd = d_tile_max = d_tile_face = nullptr;
+    U = V1 = V2 = W1 = W2 = dose_hi = dose_lo = bnd = d_tile_max = d_tile_face = nullptr;
     ids = nullptr;
     spatial = nullptr;
     d_flags[0] = d_flags[1] = nullptr;
@@ -452,6 +454,8 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         dev_alloc(&c->U, 4 * N, "hipMalloc batch state");
         dev_alloc(&c->V1, 4 * N, "hipMalloc batch stage 1");
         dev_alloc(&c->V2, 4 * N, "hipMalloc batch stage 2");
+        dev_alloc(&c->W1, 4 * N, "hipMalloc batch stage 1 input");
+        dev_alloc(&c->W2, 4 * N, "hipMalloc batch stage 2 input");
         dev_alloc(&c->dose_hi, N, "hipMalloc batch dose");
         dev_alloc(&c->dose_lo, N, "hipMalloc batch dose");
         dev_alloc(&c->bnd, 12 * size_t(cols), "hipMalloc batch boundary");
@@ -590,7 +594,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         hip_check(hipMemsetAsync(c->d_pred, 0, T, s), "prediction reset");
         hip_check(hipMemsetAsync(c->d_wmax, 0, T * sizeof(float), s), "window max reset");
         hip_check(hipMemsetAsync(c->d_cand, 0, T, s), "candidate reset");
-        for (float* p : {c->U, c->V1, c->V2})
+        for (float* p : {c->U, c->V1, c->V2, c->W1, c->W2})
             hip_check(hipMemsetAsync(p, 0, 4 * N * sizeof(float), s), "state reset");
         hip_check(hipMemsetAsync(c->dose_hi, 0, N * sizeof(float), s), "dose reset");
         hip_check(hipMemsetAsync(c->dose_lo, 0, N * sizeof(float), s), "dose reset");
@@ -661,6 +665,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 persist(0);
                 a.src = c->U;
                 a.out = c->V1;
+                a.win = nullptr;
+                a.wout = c->W1;
                 a.dose_hi = a.dose_lo = nullptr;
                 a.bnd_stage = 0;
                 Timed t(c->prof, s, K_STAGE0);
@@ -670,6 +676,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 persist(1);
                 a.src = c->V1;
                 a.out = c->V2;
+                a.win = c->W1;
+                a.wout = c->W2;
                 a.bnd_stage = 1;
                 Timed t(c->prof, s, K_STAGE1);
                 hip_check(c->K->stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
@@ -678,6 +686,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 persist(2);
                 a.src = c->V2;
                 a.out = nullptr;
+                a.win = c->W2;
+                a.wout = nullptr;
                 a.dose_hi = dhi;
                 a.dose_lo = dlo;
                 a.bnd_stage = 2;
@@ -733,7 +743,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                           "tile compaction kernel");
             }
             BatchZeroArgs za{c->d_box,         c->d_tile_box, int(T), c->d_flags[k],
-                             c->d_flags[1 - k], c->U,         c->V1,  c->V2};
+                             c->d_flags[1 - k], c->U,         c->V1,  c->V2, c->W1, c->W2};
             Timed t(c->prof, s, K_TILE_ZERO);
             hip_check(c->K->zero(za, s), "tile zero kernel (batch)");
         };
@@ -992,7 +1002,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 stages(c->d_list_full, c->d_count_full, n_full, c->jf_hi, c->jf_lo);
                 slices(SLICE_RESTORE, d_save);
                 BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags_full,
-                                 c->d_flags[cur], c->U, c->V1, c->V2};
+                                 c->d_flags[cur], c->U, c->V1, c->V2, c->W1, c->W2};
                 hip_check(c->K->zero(za, s), "judge tile reset");
                 a.judge = d_judge;
                 a.judge_hi = c->js_hi;

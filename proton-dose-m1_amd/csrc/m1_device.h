@@ -278,6 +278,10 @@ struct BatchStageArgs {
     // handed out), stealing from the other slices once their own is empty.
     const int* count;
     int* queue;
+    // Stage 1 / 2 stencil input formed by the previous stage (win), and the
+    // one this stage forms for the next (wout); nullable.
+    const float* win;
+    float* wout;
 };
 
 // Per list: its length followed by one set of XCD queue counters per stage
@@ -429,6 +433,8 @@ struct BatchZeroArgs {
     float* U;
     float* V1;
     float* V2;
+    float* W1;  // the stage inputs formed by stages 0 and 1 (nullable)
+    float* W2;
 };
 
 // Launchers of the batched kernels compiled for one tile shape.

@@ -34,6 +34,12 @@ struct TileView {
     float* tile_face;  // this tile's FACE_SLOTS face slabs (nullable; needs tile_max)
     int wave_mode;
     int tx, ty, tz;    // tile coordinates (x: TX planes, y: TY rows, z: TZ cells)
+    // Stages 1 and 2 (nullable): the stencil input proj(C_k V_k / S_k),
+    // already formed by the previous stage, read in place of src.
+    const float* win;
+    // Stages 0 and 1 (nullable): this stage's output as the next stage's
+    // stencil input, proj(C_k out / S_k), for its win.
+    float* wout;
 };
 
 // Waves per SIMD the stage kernels are compiled for (register budget:
@@ -444,10 +450,18 @@ __device__ __forceinline__ void face_hll_yz(bool zaxis, const float L[4], const 
 }
 
 // Stencil input of one cell: U directly, or proj(C_k V_k / S_k) for stages 1, 2.
-template <int KIND, bool D>
+// PRE: stages 1 and 2 read the input the previous stage formed (a.win).
+template <int KIND, bool D, bool PRE>
 __device__ __forceinline__ void load_state(const TileView& a, const StepCoef32& c, long i,
                                            long N, float u[4])
 {
+    if constexpr (PRE && (KIND == STAGE1 || KIND == STAGE2)) {
+        u[0] = a.win[i];
+        u[1] = a.win[i + N];
+        u[2] = a.win[i + 2 * N];
+        u[3] = a.win[i + 3 * N];
+        return;
+    }
     u[0] = a.src[i];
     u[1] = a.src[i + N];
     u[2] = a.src[i + 2 * N];
@@ -656,7 +670,26 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
     lo += err;
 }
 
-template <int KIND, bool IFAC, bool D>
+// The next stage's stencil input of one cell from this stage's stored
+// output v: proj(g v), g = C_k / S_k of the cell's material -- load_state's
+// arithmetic on the same floats, so the next stage reads what it would form.
+template <bool D>
+__device__ __forceinline__ void store_next_input(float* w, long i, long N, const float v[4],
+                                                 float gp, float gj)
+{
+    float u[4] = {v[0] * gp, v[1] * gj, v[2] * gj, v[3] * gj};
+    if constexpr (D)
+        project_f(u);
+    else
+        project_ff(u);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        w[i + c * N] = u[c];
+}
+
+// PRE: stages 0 and 1 also store the next stage's input (a.wout) and stages
+// 1 and 2 read theirs (a.win) instead of forming it from src.
+template <int KIND, bool IFAC, bool D, bool PRE = false>
 __device__ __forceinline__ void stage_tile(const TileView& a)
 {
     __shared__ StepCoef32 sc;
@@ -721,7 +754,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     float W[6][4];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-        load_state<KIND, D>(a, sc, cidx(clampx(x0 - 3 + k), yc, zc), N, W[k]);
+        load_state<KIND, D, PRE>(a, sc, cidx(clampx(x0 - 3 + k), yc, zc), N, W[k]);
     float Flo[4];
     {
         float L[4], R[4], up;
@@ -747,7 +780,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
     const bool h_high = hz_lane ? tid >= TY : tid - 2 * TY >= TZ;
 
     for (int x = x0; x < x1; ++x) {
-        load_state<KIND, D>(a, sc, cidx(clampx(x + 3), yc, zc), N, W[5]);
+        load_state<KIND, D, PRE>(a, sc, cidx(clampx(x + 3), yc, zc), N, W[5]);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
             P[c][ty + 3][tz + 3] = W[2][c];
@@ -757,7 +790,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 float q[4][4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    load_state<KIND, D>(a, sc, cidx(x, hgy, k), N, q[k]);
+                    load_state<KIND, D, PRE>(a, sc, cidx(x, hgy, k), N, q[k]);
                 const long b = long(x) * ny + hgy;
                 const long m2 = long(nx) * ny;
 #pragma unroll
@@ -765,7 +798,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     u[c] = zmin_ghost(-hgz_raw, a.bnd[c * m2 + b], q[0][c], q[1][c], q[2][c],
                                       q[3][c]);
             } else {
-                load_state<KIND, D>(a, sc, cidx(x, hgy, hgz), N, u);
+                load_state<KIND, D, PRE>(a, sc, cidx(x, hgy, hgz), N, u);
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -883,9 +916,14 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     a.out[i + c * N] = rhs[c];
             } else if constexpr (KIND == STAGE0) {
                 const float S0 = sc.S0[m];
+                float v[4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = fmaf(rhs[c], dE, S0 * W[2][c]);
+                for (int c = 0; c < 4; ++c) {
+                    v[c] = fmaf(rhs[c], dE, S0 * W[2][c]);
+                    a.out[i + c * N] = v[c];
+                }
+                if constexpr (PRE)
+                    store_next_input<D>(a.wout, i, N, v, sc.g1p[m], sc.g1j[m]);
             } else if constexpr (KIND == STAGE_FE) {
                 const float S0 = sc.S0[m];
                 float u[4];
@@ -906,14 +944,17 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     two_sum_add(a.dose_hi[i], a.dose_lo[i], e * sc.dose_scale[m]);
             } else if constexpr (KIND == STAGE1) {
                 const float S0 = sc.S0[m];
+                float v[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     float r = rhs[c];
                     if constexpr (IFAC)
                         r *= (c == 0 ? sc.i1p[m] : sc.i1j[m]);
-                    a.out[i + c * N] =
-                        fmaf(0.75f, S0 * a.U[i + c * N], 0.25f * fmaf(r, dE, a.V1[i + c * N]));
+                    v[c] = fmaf(0.75f, S0 * a.U[i + c * N], 0.25f * fmaf(r, dE, a.V1[i + c * N]));
+                    a.out[i + c * N] = v[c];
                 }
+                if constexpr (PRE)
+                    store_next_input<D>(a.wout, i, N, v, sc.g2p[m], sc.g2j[m]);
             } else {  // STAGE2
                 const float S0 = sc.S0[m], S2 = sc.S2[m];
                 float un[4], u[4];
