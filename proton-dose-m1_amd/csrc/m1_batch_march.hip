@@ -24,9 +24,16 @@
 // the tile stays at or below the threshold for the next `horizon` steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "m1_host.h"
@@ -43,6 +50,100 @@ namespace {
 #define M1_BATCH_TABLE_ENTRY(a, b, c) &M1_BATCH_TABLE_NAME(a, b, c),
 const BatchKernels& (*const kKernelSets[])() = {M1_BATCH_SHAPES(M1_BATCH_TABLE_ENTRY)};
 constexpr int kKernelSetCount = int(sizeof(kKernelSets) / sizeof(kKernelSets[0]));
+}  // namespace
+
+namespace {
+
+// Host threads for the per-beamlet step scalars, which sit between the CFL
+// read-back and the next launches.  parallel_for(n, f) runs f(0..n-1) on the
+// calling thread and the workers and returns when all are done; an exception
+// of any f is rethrown in the caller.
+class HostPool {
+public:
+    HostPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int n = int(std::min(hw > 1 ? hw - 1 : 0u, 15u));
+        for (int i = 0; i < n; ++i)
+            workers_.emplace_back([this] { work(); });
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (auto& t : workers_)
+            t.join();
+    }
+    void parallel_for(int n, const std::function<void(int)>& f)
+    {
+        if (workers_.empty() || n < 16) {
+            for (int i = 0; i < n; ++i)
+                f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            job_ = &f;
+            n_ = n;
+            next_.store(0);
+            busy_ = int(workers_.size());
+            error_ = nullptr;
+            ++gen_;
+        }
+        wake_.notify_all();
+        run();
+        std::unique_lock<std::mutex> l(mu_);
+        done_.wait(l, [this] { return busy_ == 0; });
+        job_ = nullptr;
+        if (error_)
+            std::rethrow_exception(error_);
+    }
+
+private:
+    void run()
+    {
+        for (int i; (i = next_.fetch_add(1)) < n_;) {
+            try {
+                (*job_)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> l(mu_);
+                if (!error_)
+                    error_ = std::current_exception();
+            }
+        }
+    }
+    void work()
+    {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                wake_.wait(l, [&] { return stop_ || gen_ != seen; });
+                if (stop_)
+                    return;
+                seen = gen_;
+            }
+            run();
+            std::lock_guard<std::mutex> l(mu_);
+            if (--busy_ == 0)
+                done_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable wake_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    int n_ = 0;
+    std::atomic<int> next_{0};
+    int busy_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr error_;
+};
+
 }  // namespace
 
 struct m1_batch {
@@ -77,6 +178,7 @@ struct m1_batch {
     int* d_count = nullptr;  // LIST_HEADER ints: length, stage queues
     bool persistent = false; // stage kernels on a fixed grid with XCD queues
     bool cfl_all_tiles = false;  // CFL over every tile of the boxes (else the live list)
+    HostPool pool;               // host threads of the per-beamlet step scalars
     int persistent_grid = 0; // resident workgroups of a stage kernel
     int cus = 0;             // compute units of the device
     bool use_graphs = false; // capture plain steps as HIP graphs
@@ -751,11 +853,12 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // flags[k], read back with the length of the list built from it.
         auto fill_cfl_host = [&]() {
             const auto t0 = std::chrono::steady_clock::now();
-            for (int b = 0; b < B; ++b)
+            c->pool.parallel_for(int(B), [&](int b) {
                 if (c->h_step[b].active) {
                     fill_cfl(c->mats, beam_params(c->bl[size_t(b)]), E[size_t(b)], c->h_cfl[b]);
                     set_norm(c->alpha[size_t(b)], c->h_cfl[b]);
                 }
+            });
             c->host_ms += ms_since(t0);
         };
         auto cfl_launch = [&](int k) {
@@ -893,9 +996,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
 
             // 1. Step scalars of every active beamlet.
             auto t0 = std::chrono::steady_clock::now();
-            for (int b = 0; b < B; ++b) {
+            c->pool.parallel_for(int(B), [&](int b) {
                 if (!c->h_step[b].active)
-                    continue;
+                    return;
                 double mx;
                 std::memcpy(&mx, &c->h_red[b], sizeof mx);
                 if (!vacuum_aware) {
@@ -925,8 +1028,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                           false, c->h_coef[b]);
                 set_norm(c->alpha[size_t(b)], c->h_coef[b]);
                 fill_step32(nm, c->alpha[size_t(b)], c->h_coef[b], c->h_coef32[b]);
-                c->prof.tiles_full += tiles_of[size_t(b)];
-            }
+            });
+            for (int b = 0; b < B; ++b)
+                if (c->h_step[b].active)
+                    c->prof.tiles_full += tiles_of[size_t(b)];
             c->host_ms += ms_since(t0);
             // Beamlets judged this step: every judge_every steps after their
             // checkpoint, and on their last step.
