@@ -49,23 +49,29 @@ MEV = 1.0
 #   stage 1: read V1 (32) + U (32) + ids (1), write V2 (32)
 #   stage 2: read V2 (32) + U (32) + V1.phi (8) + ids (1), write U (32), dose RMW (16)
 STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
-# FP32-bulk: float state (16 B per cell-state), dose as a compensated float pair:
-#   stage 0: read U (16) + ids (1), write V1 (16)
-#   stage 1: read V1 (16) + U (16) + ids (1), write V2 (16)
-#   stage 2: read V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16), dose pair RMW (16)
-STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 69}
+# FP32-bulk batch: float state (16 B per cell-state), dose as a compensated
+# float pair, the next stage's projected stencil input W formed by stages 0, 1:
+#   stage 0: read U (16) + ids (1), write V1 (16) + W1 (16)
+#   stage 1: read W1 (16) + V1 (16) + U (16) + ids (1), write V2 (16) + W2 (16)
+#   stage 2: read W2 (16) + V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16),
+#            dose pair RMW (16)
+STAGE_BYTES_FP32 = {"stage0": 49, "stage1": 81, "stage2": 85}
+# FP32-bulk single box (stages 1, 2 form their input at load, no W):
+#   stage 0: read U + ids, write V1; stage 1: read V1 + U + ids, write V2;
+#   stage 2: read V2 + U + V1.phi + ids, write U, dose pair RMW
+STAGE_BYTES_FP32_SINGLE = {"stage0": 33, "stage1": 49, "stage2": 69}
 # HBM bytes per evaluated cell of the batched stage kernels per tile shape,
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters
 # on a kernel of known traffic (profiles/stage_kernels_batch_pmc_traffic.md);
 # shapes without a measurement report traffic null.
 PMC_BYTES_PER_CELL_FP32 = {
-    (8, 16, 16): {"stage0": 66.2, "stage1": 93.3, "stage2": 114.0},
+    (8, 16, 16): {"stage0": 87.6, "stage1": 132.4, "stage2": 136.9},
 }
 # Fraction of the VALU issue slots the batched stage kernels use, per tile
 # shape (rocprofv3 SQ_INSTS_VALU x 4 cycles / (kernel time x 2.4 GHz x 1024
 # SIMDs), profiles/stage_kernels_batch_valu.md): the bound of these kernels.
 VALU_ISSUE_FRAC_FP32 = {
-    (8, 16, 16): {"stage0": 0.89, "stage1": 0.78, "stage2": 0.74},
+    (8, 16, 16): {"stage0": 0.87, "stage1": 0.77, "stage2": 0.75},
 }
 
 
@@ -454,7 +460,7 @@ def run_single(args):
     ctx = _native.Context(0)
     ctx.set_problem(grid, mm, inflow, cfg.beam)
     num = _native.numerics_struct(cfg)
-    stage_bytes = STAGE_BYTES_FP32 if precision is Precision.FP32_BULK else STAGE_BYTES_FP64
+    stage_bytes = STAGE_BYTES_FP32_SINGLE if precision is Precision.FP32_BULK else STAGE_BYTES_FP64
 
     # Plain FP64 march of the same beamlet: the accuracy reference of the
     # measured march and the denominator of its speed-up (outside the timed region).

@@ -256,18 +256,28 @@ __global__ void __launch_bounds__(NT) cfl_list_kernel(BatchCflArgs a)
 
 // Live-list form, tiles outside the live set (exact zero state), one thread
 // per tile: the tile's materials go to its box's mask, whose vacuum values
-// the host adds (an atomic only when the mask still lacks a bit).
+// the host adds.  A box's tiles are contiguous, so a wave almost always
+// covers one box: its lanes' masks are OR-reduced and one lane updates the
+// box (an atomic only when the mask still lacks a bit).
 __global__ void __launch_bounds__(256) cfl_vacuum_tiles_kernel(BatchCflArgs a)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= a.n_tiles)
-        return;
-    const int b = a.tile_box[t];
-    if (!a.step[b].active || a.live[t])
-        return;
-    const unsigned long long m = a.tile_mats[t];
-    if ((a.box_mats[b] & m) != m)
+    int b = -1;
+    unsigned long long m = 0;
+    if (t < a.n_tiles) {
+        b = a.tile_box[t];
+        if (a.step[b].active && !a.live[t])
+            m = a.tile_mats[t];
+    }
+    const int b0 = __shfl(b, 0, 64);
+    if (__all(b == b0 || b < 0)) {
+        for (int off = 32; off > 0; off >>= 1)
+            m |= __shfl_xor(m, off, 64);
+        if ((threadIdx.x & 63) == 0 && b0 >= 0 && m && (a.box_mats[b0] & m) != m)
+            atomicOr(a.box_mats + b0, m);
+    } else if (m && (a.box_mats[b] & m) != m) {
         atomicOr(a.box_mats + b, m);
+    }
 }
 
 // Live-list form, z-min columns of the tiles outside the live set, one
