@@ -532,14 +532,14 @@ __global__ void compare_kernel(BatchCompareArgs a)
         return;
     const BoxDesc d = a.box[b];
     const long n = long(d.nx) * d.ny * d.nz;
-    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     float df = 0.f, fl = 0.f;
-    if (i < n) {
+    for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += long(gridDim.x) * blockDim.x) {
         const long c = d.cell_off + i;
         const float f = a.full_hi[c] + a.full_lo[c];
         const float k = a.skip_hi[c] + a.skip_lo[c];
-        df = fabsf(f - k);
-        fl = fabsf(f);
+        df = fmaxf(df, fabsf(f - k));
+        fl = fmaxf(fl, fabsf(f));
     }
     for (int off = 32; off > 0; off >>= 1) {
         df = fmaxf(df, __shfl_xor(df, off, 64));
@@ -562,6 +562,11 @@ __global__ void compare_kernel(BatchCompareArgs a)
     }
 }
 
+__device__ __forceinline__ void slice_cell(const BatchSliceArgs& a, const BoxDesc& d, int sel,
+                                           long n, long i);
+
+// One grid row per box; the ~4 judged boxes of a step are copied by the
+// workgroups of their rows, the others' rows exit at once.
 __global__ void slices_kernel(BatchSliceArgs a)
 {
     const int b = blockIdx.y;
@@ -570,9 +575,14 @@ __global__ void slices_kernel(BatchSliceArgs a)
         return;
     const BoxDesc d = a.box[b];
     const long n = long(d.nx) * d.ny * d.nz;
-    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += long(gridDim.x) * blockDim.x)
+        slice_cell(a, d, sel, n, i);
+}
+
+__device__ __forceinline__ void slice_cell(const BatchSliceArgs& a, const BoxDesc& d, int sel,
+                                           long n, long i)
+{
     const long c = d.cell_off + i;
     switch (a.op) {
     case SLICE_SAVE: {
@@ -831,8 +841,8 @@ hipError_t launch_compare(const BatchCompareArgs& a, int max_cells, hipStream_t 
 {
     if (a.n_boxes <= 0 || max_cells <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(compare_kernel, dim3(unsigned((max_cells + 255) / 256), unsigned(a.n_boxes)),
-                       dim3(256), 0, s, a);
+    const unsigned gx = unsigned(std::min((max_cells + 255) / 256, 64));
+    hipLaunchKernelGGL(compare_kernel, dim3(gx, unsigned(a.n_boxes)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -855,8 +865,8 @@ hipError_t launch_slices(const BatchSliceArgs& a, int max_cells, hipStream_t s)
 {
     if (a.n_boxes <= 0 || max_cells <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(slices_kernel, dim3(unsigned((max_cells + 255) / 256), unsigned(a.n_boxes)),
-                       dim3(256), 0, s, a);
+    const unsigned gx = unsigned(std::min((max_cells + 255) / 256, 64));
+    hipLaunchKernelGGL(slices_kernel, dim3(gx, unsigned(a.n_boxes)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
