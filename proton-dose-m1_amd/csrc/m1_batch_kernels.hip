@@ -621,8 +621,10 @@ __device__ __forceinline__ void slice_cell(const BatchSliceArgs& a, const BoxDes
 }
 
 // One workgroup: ascending list of the set flags (each thread scans one
-// contiguous chunk; chunk counts are prefix-summed in LDS).
+// contiguous chunk; chunk counts are prefix-summed in LDS).  Used when the
+// chunk counts of the multi-workgroup form would not fit after the header.
 constexpr int COMPACT_THREADS = 1024;
+constexpr int COMPACT_CHUNK = 1024;
 
 __global__ void __launch_bounds__(COMPACT_THREADS)
     compact_kernel(const uint8_t* flags, int n, int* list, int* count)
@@ -652,12 +654,105 @@ __global__ void __launch_bounds__(COMPACT_THREADS)
         *count = sums[tid];
 }
 
+// Multi-workgroup form: workgroup g takes flags [g C, (g + 1) C), C = 1024,
+// four consecutive flags per thread.  Pass 1 writes each chunk's count after
+// the list header; pass 2 adds the counts of the chunks before its own and
+// writes its entries in order, so the list is the same ascending one.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* wsum, int& total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int k = 0; k < 256 / 64; ++k) {
+        if (k < w)
+            before += wsum[k];
+        total += wsum[k];
+    }
+    return before + x - v;
+}
+
+__device__ __forceinline__ int chunk_flags(const uint8_t* flags, int n, int i0, bool f[4])
+{
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[k] = i0 + k < n && flags[i0 + k] != 0;
+        c += f[k];
+    }
+    return c;
+}
+
+__global__ void __launch_bounds__(256)
+    compact_count_kernel(const uint8_t* flags, int n, int* count)
+{
+    __shared__ int wsum[256 / 64];
+    if (blockIdx.x == 0 && threadIdx.x < 3 * XCD_QUEUES)
+        count[1 + threadIdx.x] = 0;  // the stage queues of this list
+    bool f[4];
+    const int c = chunk_flags(flags, n, blockIdx.x * COMPACT_CHUNK + 4 * threadIdx.x, f);
+    int total;
+    block_exclusive_scan(c, wsum, total);
+    if (threadIdx.x == 0)
+        count[LIST_HEADER + blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256)
+    compact_write_kernel(const uint8_t* flags, int n, int* list, int* count)
+{
+    __shared__ int wsum[256 / 64];
+    __shared__ int s_off;
+    const int* chunk = count + LIST_HEADER;
+    int off = 0;
+    for (int j = threadIdx.x; j < int(blockIdx.x); j += 256)
+        off += chunk[j];
+    for (int d = 32; d > 0; d >>= 1)
+        off += __shfl_xor(off, d, 64);
+    if ((threadIdx.x & 63) == 0)
+        wsum[threadIdx.x >> 6] = off;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_off = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int base = s_off;
+    __syncthreads();  // wsum is reused by the scan
+    bool f[4];
+    const int i0 = blockIdx.x * COMPACT_CHUNK + 4 * threadIdx.x;
+    const int c = chunk_flags(flags, n, i0, f);
+    int total;
+    int pos = base + block_exclusive_scan(c, wsum, total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (f[k])
+            list[pos++] = i0 + k;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        *count = base + total;
+}
+
 // Tiles that leave the evaluated set become exact vacuum (U = V1 = V2 = 0).
+__device__ __forceinline__ void zero_tile(const BatchZeroArgs& a, int t);
+
+// A grid of at most ZERO_GRID workgroups walks the tiles; each zeroes the
+// leaving ones it meets.
+constexpr int ZERO_GRID = 4096;
+
 __global__ void __launch_bounds__(NT) zero_kernel(BatchZeroArgs a)
 {
-    const int t = blockIdx.x;
-    if (!(a.computed[t] && !a.next[t]))
-        return;
+    for (int t = blockIdx.x; t < a.n_tiles; t += gridDim.x)
+        if (a.computed[t] && !a.next[t])
+            zero_tile(a, t);
+}
+
+__device__ __forceinline__ void zero_tile(const BatchZeroArgs& a, int t)
+{
     const int b = a.tile_box[t];
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
@@ -798,8 +893,16 @@ hipError_t launch_mask(const BatchMaskArgs& a, hipStream_t s)
 
 hipError_t launch_compact(const uint8_t* flags, int n, int* list, int* count, hipStream_t s)
 {
-    hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(COMPACT_THREADS), 0, s, flags, n, list,
-                       count);
+    const int blocks = (n + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+    if (blocks >= 1 && blocks <= COMPACT_MAX_BLOCKS) {
+        hipLaunchKernelGGL(compact_count_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, flags,
+                           n, count);
+        hipLaunchKernelGGL(compact_write_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, flags,
+                           n, list, count);
+    } else {
+        hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(COMPACT_THREADS), 0, s, flags, n, list,
+                           count);
+    }
     return hipGetLastError();
 }
 
@@ -807,7 +910,8 @@ hipError_t launch_zero(const BatchZeroArgs& a, hipStream_t s)
 {
     if (a.n_tiles <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(a.n_tiles)), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min(a.n_tiles, ZERO_GRID))), dim3(NT), 0,
+                       s, a);
     return hipGetLastError();
 }
 

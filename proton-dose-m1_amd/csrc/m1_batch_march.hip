@@ -175,7 +175,7 @@ struct m1_batch {
     double* spatial = nullptr;
     uint8_t* d_flags[2] = {nullptr, nullptr};
     int* d_list = nullptr;
-    int* d_count = nullptr;  // LIST_HEADER ints: length, stage queues
+    int* d_count = nullptr;  // LIST_HEADER_ALLOC ints: length, stage queues, chunk counts
     bool persistent = false; // stage kernels on a fixed grid with XCD queues
     bool cfl_all_tiles = false;  // CFL over every tile of the boxes (else the live list)
     HostPool pool;               // host threads of the per-beamlet step scalars
@@ -568,7 +568,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         dev_alloc(&c->d_flags[0], T, "hipMalloc tile set");
         dev_alloc(&c->d_flags[1], T, "hipMalloc tile set");
         dev_alloc(&c->d_list, T, "hipMalloc tile list");
-        dev_alloc(&c->d_count, LIST_HEADER, "hipMalloc tile count");
+        dev_alloc(&c->d_count, LIST_HEADER_ALLOC, "hipMalloc tile count");
         dev_alloc(&c->d_coef, B, "hipMalloc batch coefficients");
         dev_alloc(&c->d_cfl, B, "hipMalloc batch CFL coefficients");
         dev_alloc(&c->d_coef32, B, "hipMalloc batch coefficients (f32)");
@@ -664,7 +664,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             dev_alloc(&c->d_flags_full, T, "hipMalloc judge tile set");
             dev_alloc(&c->d_list_full, T, "hipMalloc judge tile list");
             hip_check(hipMemset(c->d_list_full, 0, T * sizeof(int)), "judge tile list");
-            dev_alloc(&c->d_count_full, LIST_HEADER, "hipMalloc judge tile count");
+            dev_alloc(&c->d_count_full, LIST_HEADER_ALLOC, "hipMalloc judge tile count");
             dev_alloc(&c->d_jdiff, nb, "hipMalloc judge maxima");
             dev_alloc(&c->d_jfull, nb, "hipMalloc judge maxima");
             host_alloc(&c->h_jdiff, nb, "hipHostMalloc");

@@ -288,6 +288,9 @@ struct BatchStageArgs {
 // (the compaction that writes the list clears them).
 constexpr int XCD_QUEUES = 8;
 constexpr int LIST_HEADER = 1 + 3 * XCD_QUEUES;
+// The compaction's per-chunk counts follow the header (chunks of 1024 flags).
+constexpr int COMPACT_MAX_BLOCKS = 4096;
+constexpr int LIST_HEADER_ALLOC = LIST_HEADER + COMPACT_MAX_BLOCKS;
 
 struct BatchCflArgs {
     const BoxDesc* box;
