@@ -65,13 +65,13 @@ STAGE_BYTES_FP32_SINGLE = {"stage0": 33, "stage1": 49, "stage2": 69}
 # on a kernel of known traffic (profiles/stage_kernels_batch_pmc_traffic.md);
 # shapes without a measurement report traffic null.
 PMC_BYTES_PER_CELL_FP32 = {
-    (8, 16, 16): {"stage0": 87.6, "stage1": 132.4, "stage2": 136.9},
+    (8, 16, 16): {"stage0": 87.6, "stage1": 132.3, "stage2": 136.8},
 }
 # Fraction of the VALU issue slots the batched stage kernels use, per tile
 # shape (rocprofv3 SQ_INSTS_VALU x 4 cycles / (kernel time x 2.4 GHz x 1024
 # SIMDs), profiles/stage_kernels_batch_valu.md): the bound of these kernels.
 VALU_ISSUE_FRAC_FP32 = {
-    (8, 16, 16): {"stage0": 0.87, "stage1": 0.77, "stage2": 0.75},
+    (8, 16, 16): {"stage0": 0.86, "stage1": 0.77, "stage2": 0.75},
 }
 
 
