@@ -8,6 +8,8 @@
 // list of (box, tile) pairs, and workgroups are mapped so that each of the 8
 // XCDs (one L2 each) takes one contiguous slice of the list, keeping the
 // halo re-reads of neighbouring tiles in the same L2.
+#include <atomic>
+
 #include "m1_f32_common.h"
 #include "m1_mlp.h"
 
@@ -857,14 +859,18 @@ hipError_t launch_cfl(const BatchCflArgs& a, hipStream_t s)
             hipLaunchKernelGGL(cfl_inflow_columns_kernel,
                                dim3(unsigned((a.max_columns + 255) / 256), unsigned(a.n_boxes)),
                                dim3(256), 0, s, a);
-        // a grid of resident workgroups walking the live list
-        static int grid = 0;
+        // a grid of resident workgroups walking the live list (per device)
+        static std::atomic<int> grids[64];
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        int grid = dev >= 0 && dev < 64 ? grids[dev].load() : 0;
         if (grid == 0) {
-            int dev = 0, cus = 0, per = 0;
-            (void)hipGetDevice(&dev);
+            int cus = 0, per = 0;
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, cfl_list_kernel, NT, 0);
             grid = (cus > 0 ? cus : 256) * (per > 0 ? per : 4);
+            if (dev >= 0 && dev < 64)
+                grids[dev].store(grid);
         }
         hipLaunchKernelGGL(cfl_list_kernel, dim3(unsigned(min(grid, a.n_tiles))), dim3(NT), 0,
                            s, a);
