@@ -4,7 +4,10 @@
  * The library replaces the hot path of anis-el-hachmioui/proton-dose-m1:
  * src/proton_solver/numerics/energy.py:1497-1644 (energy_march) and its
  * callees (spatial.py:104-252, boundary.py:117-399, weno_z.py:13-321,
- * flux.py:10-386, wave_speeds.py:12-105, realizability.py:132-215).
+ * flux.py:10-386, wave_speeds.py:12-105, realizability.py:132-215), the
+ * moving active-domain march (energy.py:1262-1494 with
+ * moving_active_domain.py:35-253) and the vacuum-aware CFL
+ * (energy.py:459-537).
  * Signatures use plain C types only: the Python drop-in binds them with
  * ctypes (proton_solver/_native.py); any other FFI can bind them the same way.
  *
