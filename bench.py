@@ -436,14 +436,15 @@ def main():
     if args.workload == "batch256":
         args.steps = 1 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
-        args.cpu_sample_steps = 20 if args.cpu_sample_steps is None else args.cpu_sample_steps
+        # about 10 s of oracle work on the host's cores
+        args.cpu_sample_steps = 100 if args.cpu_sample_steps is None else args.cpu_sample_steps
         if args.no_fp64_reference:
             args.fp64_sample = 0
         run_batch(args)
         return
     args.steps = 3 if args.steps is None else args.steps
     args.warmup = 1 if args.warmup is None else args.warmup
-    args.cpu_sample_steps = 4 if args.cpu_sample_steps is None else args.cpu_sample_steps
+    args.cpu_sample_steps = 20 if args.cpu_sample_steps is None else args.cpu_sample_steps
     run_single(args)
 
 
