@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 MEV = 1.0
 
 # Algorithmic HBM bytes per cell of each FP64 stage kernel: every stage input
-# read once, every output written once, material id 1 byte (see DESIGN.md):
+# read once, every output written once, material id 1 byte:
 #   stage 0: read U (32) + ids (1), write V1 (32)
 #   stage 1: read V1 (32) + U (32) + ids (1), write V2 (32)
 #   stage 2: read V2 (32) + U (32) + V1.phi (8) + ids (1), write U (32), dose RMW (16)
@@ -429,8 +429,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (beamlet "
-              "sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one MI355X: there is no multi-GPU path (beamlets are "
+              "independent solves and would be sharded across GPUs by cost).",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch256":

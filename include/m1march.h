@@ -255,7 +255,8 @@ int m1_reset_stats(m1_ctx *ctx);
 /* Tiles evaluated by the stage kernels since the last m1_reset_stats, and the
  * number a full-domain march would have evaluated. */
 int m1_tile_stats(m1_ctx *ctx, int64_t *evaluated, int64_t *full);
-/* Cells per FP32-bulk workgroup tile along x, y and z in this build. */
+/* Cells per FP32-bulk workgroup tile along x, y and z of the single-box
+ * kernels (fixed when the library is compiled). */
 void m1_tile_shape(int32_t *tx, int32_t *ty, int32_t *tz);
 
 /* ------------------------------------------------------------------
