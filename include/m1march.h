@@ -218,7 +218,8 @@ int m1_compute_energy_step_vacuum(m1_ctx *ctx, const m1_numerics *num, double en
  * reference are host (nx, ny, nz) arrays; device memory is allocated and
  * freed inside the call.  The arithmetic is that of
  * proton_solver.metrics.gamma_pass_rate, so the count equals the host
- * search's.  m1_gamma_last_error gives the message of a failed call. */
+ * search's.  m1_gamma_last_error gives the message of the calling thread's
+ * last failed call. */
 int m1_gamma_pass_count(int32_t device, const double *dose, const double *reference,
                         int32_t nx, int32_t ny, int32_t nz, const int32_t *points,
                         int32_t n_points, const double *offsets, const double *dist_terms,

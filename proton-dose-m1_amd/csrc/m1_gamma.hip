@@ -21,7 +21,7 @@ namespace {
 
 constexpr int THREADS = 256;
 
-std::string g_gamma_error;
+thread_local std::string g_gamma_error;  // per calling thread
 
 __global__ void __launch_bounds__(THREADS)
 gamma_kernel(const double* __restrict__ dose, const double* __restrict__ ref, int nx, int ny,

@@ -136,6 +136,26 @@ def batch_workload(n_beamlets=256, phantom_seed=4, beamlet_seed=40, shape=(200, 
     return phantom, beamlets
 
 
+# The five tabulated materials (data/materials.csv), in the order the
+# combined workload's ellipsoids draw them.
+FIVE_MATERIALS = (Material.AIR, Material.WATER, Material.LUNG, Material.BONE_COMPACT,
+                  Material.MUSCLE_SKELETAL)
+
+
+def combined_workload(n_beamlets=1024, phantom_seed=5, beamlet_seed=50,
+                      shape=(256, 256, 200), spacing=(0.001, 0.001, 0.001), n_ellipsoids=40):
+    """The combined CT-like workload: a 256x256x200 water phantom at 1 mm with
+    40 seeded ellipsoids (semi-axes 5-30 mm) drawn from the five tabulated
+    materials (air, water, lung 1.04, cortical bone 1.85, skeletal muscle), and
+    1024 beamlets with the 256-beamlet workload's distributions."""
+    size = tuple(n * d for n, d in zip(shape, spacing))
+    ell = random_ellipsoids(np.random.default_rng(phantom_seed), n_ellipsoids, size,
+                            materials=FIVE_MATERIALS)
+    phantom = ellipsoid_phantom(shape, spacing, ell)
+    beamlets = random_beamlets(np.random.default_rng(beamlet_seed), n_beamlets)
+    return phantom, beamlets
+
+
 # Tile threshold of the batched workloads, relative to a beamlet's peak inflow
 # fluence: tiles whose 9-cell neighbourhood stays below it are left out and
 # held at exact vacuum.  1e-6 lies below the truncation of the reference's

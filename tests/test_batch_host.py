@@ -13,8 +13,8 @@ from scipy.integrate import quad
 
 from proton_solver import Material
 from proton_solver.batch import (
-    LUNG_026, Beamlet, Ellipsoid, batch_workload, beamlet_boxes, box_depths, box_problem,
-    ellipsoid_phantom, full_problem,
+    FIVE_MATERIALS, LUNG_026, Beamlet, Ellipsoid, batch_workload, beamlet_boxes, box_depths,
+    box_problem, combined_workload, ellipsoid_phantom, full_problem,
 )
 from proton_solver.physics.materials import ScaledMaterial, density, radiation_length
 from proton_solver.physics.nuclear import load_nuclear_table
@@ -92,6 +92,38 @@ def test_batch_workload_is_seeded_and_in_range():
     assert np.abs(c).max() <= 0.030
     # uniform draws: the sample spans most of each range
     assert E.max() - E.min() > 140.0 and s.max() - s.min() > 0.0035
+
+
+def test_combined_workload_is_seeded_and_in_range():
+    """256 x 256 x 200 phantom, 40 ellipsoids over the five tabulated
+    materials, 1024 beamlets with the 256-beamlet distributions."""
+    ph, bl = combined_workload()
+    ph2, bl2 = combined_workload()
+    assert np.array_equal(ph.ids, ph2.ids) and bl == bl2
+    assert ph.shape == (256, 256, 200) and ph.spacing == (0.001, 0.001, 0.001)
+    assert set(ph.materials) == set(FIVE_MATERIALS) and ph.materials[0] is Material.WATER
+    counts = np.bincount(ph.ids.ravel(), minlength=len(ph.materials))
+    assert counts.min() > 0 and counts[0] > 0.9 * ph.ids.size
+    assert len(bl) == 1024
+    E = np.array([b.energy for b in bl])
+    c = np.array([b.center for b in bl])
+    assert 70.0 <= E.min() and E.max() <= 230.0 and np.abs(c).max() <= 0.030
+    # the first 256 beamlets differ from the 256-beamlet workload's (other seed)
+    assert [b.energy for b in bl[:256]] != [b.energy for b in batch_workload()[1]]
+
+
+def test_cfl_step_in_air_is_about_a_thousandth_of_water():
+    """Energy step of the CFL bound at the isotropic speed 1/sqrt(3) on
+    1 mm cells, dE = cfl / sum_axes(a / (S dx)), per material: air's linear
+    stopping power is 1/934 of water's at 70-230 MeV, so one active
+    (or, in the whole-phantom bound, present) air cell shrinks the step by
+    that factor."""
+    iso = 1.0 / math.sqrt(3.0)
+    for e in (70.0, 150.0, 230.0):
+        step = {m: 0.9 / (3 * iso / (stopping_power(m, e) * 0.001)) for m in FIVE_MATERIALS}
+        ratio = step[Material.WATER] / step[Material.AIR]
+        assert 930.0 < ratio < 940.0, (e, ratio)
+        assert step[Material.BONE_COMPACT] > step[Material.WATER] > step[Material.AIR]
 
 
 # ------------------------------------------------------------ boxes
