@@ -105,3 +105,68 @@ def test_workload_fp32_bulk_against_fp64(workload, closure, fp64_runs):
     assert np.all(np.isfinite(dose)) and np.all(dose >= 0)
     assert abs(md) <= MAX_DOSE_TOL
     assert gam >= GAMMA_MIN
+
+
+# Beamlets of the combined workload (1024 beamlets, 256 x 256 x 200 phantom,
+# five materials) whose box and a +-48 mm footprint around the entry point
+# hold no air over the box depth (131 of the 1024; the first six here, 73 to
+# 147 MeV).
+COMBINED_AIR_FREE = (1, 3, 19, 21, 23, 29)
+
+
+def test_combined_workload_air_free_beamlets():
+    """Six beamlets of the combined five-material workload in one batch, with
+    the double closure (FP32_BULK_F64_CLOSURE), the vacuum-aware CFL, the
+    tile-skip network and judge, and graph steps, against the FP64
+    moving-domain march with the vacuum-aware CFL (the reference's own march
+    for phantoms holding air, energy.py:1262-1494) of each beamlet on the
+    whole phantom.  The plain FP64 march is not the comparison here: any air
+    cell of the phantom bounds its step at 1/934 of water's.  Beamlets near
+    air are left out for the same reason (their steps are air-bound while the
+    beam passes it).  Observed on an MI355X: the same energy steps in both
+    (309 - 604), maximum-dose error -1.7e-4 .. 4.9e-4, gamma 100 %."""
+    from proton_solver import tileskip
+    from proton_solver.batch import (batch_workload_config, combined_workload, full_problem,
+                                     prepare_batch)
+
+    phantom, beamlets = combined_workload()
+    air = phantom.materials.index(Material.AIR)
+    chosen = [beamlets[i] for i in COMBINED_AIR_FREE]
+    cfg = batch_workload_config(vacuum_aware_cfl=True)
+    handle, boxes = prepare_batch(phantom, chosen, cfg)
+    for (x0, y0, nx, ny, nz), bl in zip(boxes, chosen):
+        ix = int(np.floor(bl.center[0] / 0.001)) + 128
+        iy = int(np.floor(bl.center[1] / 0.001)) + 128
+        foot = phantom.ids[max(ix - 48, 0):ix + 48, max(iy - 48, 0):iy + 48, :nz]
+        assert not (foot == air).any()
+    model = tileskip.load_model(0) if tileskip.WEIGHTS_FILE.exists() else None
+    tileskip.enable(handle, model, tileskip.SkipOptions())
+    handle.set_graphs(True)
+    say("combined workload: beamlets", COMBINED_AIR_FREE, "network", model is not None)
+    t0 = time.perf_counter()
+    _, steps = handle.march(_native.numerics_struct(cfg, _native.FP32_BULK_F64_CLOSURE),
+                            want_dose=False)
+    say(f"  batch {time.perf_counter() - t0:.2f} s, steps {list(steps)}")
+    handle.set_graphs(False)
+    ctx = _native.Context(0)
+    num64 = _native.numerics_struct(cfg, _native.FP64)
+    for b, bl in enumerate(chosen):
+        x0, y0, nx, ny, nz = boxes[b]
+        got = handle.beamlet_dose(b)
+        ctx.set_problem(*full_problem(phantom, bl))
+        ctx.set_state(None)
+        t0 = time.perf_counter()
+        E, _, mboxes = ctx.march_moving(num64, 9, 5.0)
+        sec = time.perf_counter() - t0
+        full = ctx.get_dose()
+        ref = full[x0:x0 + nx, y0:y0 + ny, :nz]
+        outside = 1.0 - ref.sum() / full.sum()
+        md = got.max() / full.max() - 1.0
+        gam = metrics.gamma_pass_rate(got, ref, phantom.spacing, 1.0, 1.0, 10.0, device=0)
+        say(f"  beamlet {COMBINED_AIR_FREE[b]} ({bl.energy:.1f} MeV): batch {steps[b]} steps, "
+            f"FP64 moving {len(E) - 1} steps {sec:.2f} s, dose outside the box {outside:.1e}, "
+            f"max-dose err {md:.3e}, gamma(1%/1mm) {gam:.3f} %")
+        assert np.all(np.isfinite(got)) and np.all(got >= 0)
+        assert outside < 1e-3
+        assert abs(md) <= MAX_DOSE_TOL
+        assert gam >= GAMMA_MIN
