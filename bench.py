@@ -104,6 +104,17 @@ def host_problem(cfg):
     return grid, mm, inflow
 
 
+def host_cpu_model():
+    """Model name of the host CPU (from /proc/cpuinfo), or None."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cfg, n_steps_full, sample_steps):
     """Oracle (C/OpenMP FP64 restatement) on the first energy steps of the march."""
     from oracle.oracle import OracleProblem
@@ -127,6 +138,7 @@ def cpu_baseline(cfg, n_steps_full, sample_steps):
                    f"{grid.shape[0]}x{grid.shape[1]}x{grid.shape[2]} march "
                    f"({dt:.1f} s), extrapolated to the full march"),
         "cell_steps_per_s": n * steps / dt,
+        "cpu_model": host_cpu_model(),
     }
 
 
@@ -190,6 +202,7 @@ def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_
                    f"{rate:.3g} cell-steps/s), extrapolated with the batch's mean box "
                    f"cell-steps per beamlet"),
         "cell_steps_per_s": rate,
+        "cpu_model": host_cpu_model(),
     }
 
 
