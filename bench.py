@@ -115,6 +115,23 @@ def host_cpu_model():
     return None
 
 
+def single_thread_rate(op, n_cells, steps, threads):
+    """Cell-steps/s of the same oracle march on one host thread (a shorter
+    sample), then the thread count restored."""
+    from oracle.oracle import set_threads
+
+    set_threads(1)
+    try:
+        t0 = time.perf_counter()
+        _, _, E, _ = op.march(max_steps=steps)
+        dt = time.perf_counter() - t0
+    finally:
+        set_threads(threads)
+    done = len(E) - 1
+    return {"cell_steps_per_s": n_cells * done / dt,
+            "sample": f"first {done} energy steps ({dt:.1f} s)"}
+
+
 def cpu_baseline(cfg, n_steps_full, sample_steps):
     """Oracle (C/OpenMP FP64 restatement) on the first energy steps of the march."""
     from oracle.oracle import OracleProblem
@@ -129,6 +146,7 @@ def cpu_baseline(cfg, n_steps_full, sample_steps):
     steps = len(E) - 1
     n = int(np.prod(grid.shape))
     per_step = dt / steps
+    one = single_thread_rate(op, n, max(sample_steps // 10, 1), threads)
     return {
         "value": 1.0 / (per_step * n_steps_full),
         "unit": "beamlet dose solves/s",
@@ -139,6 +157,7 @@ def cpu_baseline(cfg, n_steps_full, sample_steps):
                    f"({dt:.1f} s), extrapolated to the full march"),
         "cell_steps_per_s": n * steps / dt,
         "cpu_model": host_cpu_model(),
+        "single_thread": one,
     }
 
 
@@ -192,6 +211,7 @@ def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_
     steps = len(E) - 1
     n = int(np.prod(grid.shape))
     rate = n * steps / dt
+    one = single_thread_rate(op, n, max(sample_steps // 10, 1), threads)
     return {
         "value": rate / cell_steps_per_beamlet,
         "unit": "beamlet dose solves/s",
@@ -203,6 +223,7 @@ def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_
                    f"cell-steps per beamlet"),
         "cell_steps_per_s": rate,
         "cpu_model": host_cpu_model(),
+        "single_thread": one,
     }
 
 
