@@ -156,6 +156,26 @@ def combined_workload(n_beamlets=1024, phantom_seed=5, beamlet_seed=50,
     return phantom, beamlets
 
 
+def air_free_beamlets(phantom: Phantom, beamlets, boxes, margin_cells=48):
+    """Indices of the beamlets with no air within `margin_cells` cells of the
+    entry column (laterally) over their box depth: under the vacuum-aware CFL
+    their energy steps are not bound by air (whose step is 1/934 of water's)."""
+    if Material.AIR not in phantom.materials:
+        return list(range(len(beamlets)))
+    air = phantom.ids == phantom.materials.index(Material.AIR)
+    px, py, _ = phantom.shape
+    dx, dy, _ = phantom.spacing
+    out = []
+    for b, (bl, box) in enumerate(zip(beamlets, boxes)):
+        ix = int(np.floor((bl.center[0] + 0.5 * px * dx) / dx))
+        iy = int(np.floor((bl.center[1] + 0.5 * py * dy) / dy))
+        nz = box[4]
+        if not air[max(ix - margin_cells, 0):ix + margin_cells,
+                   max(iy - margin_cells, 0):iy + margin_cells, :nz].any():
+            out.append(b)
+    return out
+
+
 # Tile threshold of the batched workloads, relative to a beamlet's peak inflow
 # fluence: tiles whose 9-cell neighbourhood stays below it are left out and
 # held at exact vacuum.  1e-6 lies below the truncation of the reference's

@@ -13,8 +13,8 @@ from scipy.integrate import quad
 
 from proton_solver import Material
 from proton_solver.batch import (
-    FIVE_MATERIALS, LUNG_026, Beamlet, Ellipsoid, batch_workload, beamlet_boxes, box_depths,
-    box_problem, combined_workload, ellipsoid_phantom, full_problem,
+    FIVE_MATERIALS, LUNG_026, Beamlet, Ellipsoid, air_free_beamlets, batch_workload,
+    beamlet_boxes, box_depths, box_problem, combined_workload, ellipsoid_phantom, full_problem,
 )
 from proton_solver.physics.materials import ScaledMaterial, density, radiation_length
 from proton_solver.physics.nuclear import load_nuclear_table
@@ -110,6 +110,28 @@ def test_combined_workload_is_seeded_and_in_range():
     assert 70.0 <= E.min() and E.max() <= 230.0 and np.abs(c).max() <= 0.030
     # the first 256 beamlets differ from the 256-beamlet workload's (other seed)
     assert [b.energy for b in bl[:256]] != [b.energy for b in batch_workload()[1]]
+
+
+def test_air_free_beamlets_of_the_combined_workload():
+    """131 of the 1024 beamlets have no air within 48 cells of their entry
+    column over their box depth; a phantom without air keeps every beamlet."""
+    ph, bl = combined_workload()
+    boxes = beamlet_boxes(ph, bl)
+    idx = air_free_beamlets(ph, bl, boxes)
+    assert len(idx) == 131 and idx[:6] == [1, 3, 19, 21, 23, 29]
+    air = ph.ids == ph.materials.index(Material.AIR)
+    for b in idx[:20]:
+        ix = int(np.floor(bl[b].center[0] / 0.001)) + 128
+        iy = int(np.floor(bl[b].center[1] / 0.001)) + 128
+        assert not air[max(ix - 48, 0):ix + 48, max(iy - 48, 0):iy + 48, :boxes[b][4]].any()
+    # the rejected ones do have air in that footprint (the first five checked)
+    rejected = [b for b in range(20) if b not in idx][:5]
+    for b in rejected:
+        ix = int(np.floor(bl[b].center[0] / 0.001)) + 128
+        iy = int(np.floor(bl[b].center[1] / 0.001)) + 128
+        assert air[max(ix - 48, 0):ix + 48, max(iy - 48, 0):iy + 48, :boxes[b][4]].any()
+    ph4, bl4 = batch_workload(n_beamlets=8)
+    assert air_free_beamlets(ph4, bl4, beamlet_boxes(ph4, bl4)) == list(range(8))
 
 
 def test_cfl_step_in_air_is_about_a_thousandth_of_water():

@@ -126,11 +126,13 @@ def test_combined_workload_air_free_beamlets():
     beam passes it).  Observed on an MI355X: the same energy steps in both
     (309 - 604), maximum-dose error -1.7e-4 .. 4.9e-4, gamma 100 %."""
     from proton_solver import tileskip
-    from proton_solver.batch import (batch_workload_config, combined_workload, full_problem,
-                                     prepare_batch)
+    from proton_solver.batch import (air_free_beamlets, batch_workload_config, beamlet_boxes,
+                                     combined_workload, full_problem, prepare_batch)
 
     phantom, beamlets = combined_workload()
     air = phantom.materials.index(Material.AIR)
+    assert air_free_beamlets(phantom, beamlets, beamlet_boxes(phantom, beamlets))[:6] == \
+        list(COMBINED_AIR_FREE)
     chosen = [beamlets[i] for i in COMBINED_AIR_FREE]
     cfg = batch_workload_config(vacuum_aware_cfl=True)
     handle, boxes = prepare_batch(phantom, chosen, cfg)
