@@ -40,39 +40,57 @@ sys.path.insert(0, str(ROOT))
 
 import numpy as np  # noqa: E402
 
+T_START = time.perf_counter()
 PEAK_HBM_GBS = 8000.0
 MEV = 1.0
 
-# Algorithmic HBM bytes per cell of each FP64 stage kernel: every stage input
-# read once, every output written once, material id 1 byte:
+# Algorithmic HBM bytes per cell of each stage kernel: the compulsory traffic
+# of a three-stage step with perfect halo reuse (each stage input read once,
+# each output written once, 1-byte material id), split over the stages.
+# FP64 (s = 8 B per value, dose read and written):
 #   stage 0: read U (32) + ids (1), write V1 (32)
 #   stage 1: read V1 (32) + U (32) + ids (1), write V2 (32)
 #   stage 2: read V2 (32) + U (32) + V1.phi (8) + ids (1), write U (32), dose RMW (16)
 STAGE_BYTES_FP64 = {"stage0": 65, "stage1": 97, "stage2": 121}
-# FP32-bulk batch: float state (16 B per cell-state), dose as a compensated
-# float pair, the next stage's projected stencil input W formed by stages 0, 1:
-#   stage 0: read U (16) + ids (1), write V1 (16) + W1 (16)
-#   stage 1: read W1 (16) + V1 (16) + U (16) + ids (1), write V2 (16) + W2 (16)
-#   stage 2: read W2 (16) + V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16),
-#            dose pair RMW (16)
-STAGE_BYTES_FP32 = {"stage0": 49, "stage1": 81, "stage2": 85}
-# FP32-bulk single box (stages 1, 2 form their input at load, no W):
-#   stage 0: read U + ids, write V1; stage 1: read V1 + U + ids, write V2;
-#   stage 2: read V2 + U + V1.phi + ids, write U, dose pair RMW
-STAGE_BYTES_FP32_SINGLE = {"stage0": 33, "stage1": 49, "stage2": 69}
-# HBM bytes per evaluated cell of the batched stage kernels per tile shape,
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters
-# on a kernel of known traffic (profiles/stage_kernels_batch_pmc_traffic.md);
-# shapes without a measurement report traffic null.
+# FP32 bulk (s = 4) with a compensated FP64 dose pair (value and compensation
+# read and written, 32 B):
+#   stage 0: read U (16) + ids (1), write V1 (16)
+#   stage 1: read V1 (16) + U (16) + ids (1), write V2 (16)
+#   stage 2: read V2 (16) + U (16) + V1.phi (4) + ids (1), write U (16), dose pair RMW (32)
+# The batch moves more than this (its stages 0 and 1 also store the next
+# stage's projected stencil input, 32 B) and less (its compensated dose is a
+# float pair, 16 B); the measured traffic shows what it moves.
+STAGE_BYTES_FP32 = {"stage0": 33, "stage1": 49, "stage2": 85}
+STAGE_BYTES_FP32_SINGLE = STAGE_BYTES_FP32
+# Per cell-step (the three stages): 283 B in FP64, 167 B in FP32 bulk.
+STEP_BYTES_FP64 = sum(STAGE_BYTES_FP64.values())
+STEP_BYTES_FP32 = sum(STAGE_BYTES_FP32.values())
+# HBM bytes per evaluated cell of the batched stage kernels per (tile shape,
+# precision), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the
+# same counters on a kernel of known traffic (the summary named in
+# TRAFFIC_PROFILE); shapes without a measurement report traffic null.
+TRAFFIC_PROFILE = "profiles/stage_kernels_batch_pmc_traffic.md"
 PMC_BYTES_PER_CELL_FP32 = {
-    (8, 16, 16): {"stage0": 87.6, "stage1": 132.3, "stage2": 136.8},
+    ((8, 16, 16), "fp32_bulk"): {"stage0": 87.6, "stage1": 132.3, "stage2": 136.8},
 }
-# Fraction of the VALU issue slots the batched stage kernels use, per tile
-# shape (rocprofv3 SQ_INSTS_VALU x 4 cycles / (kernel time x 2.4 GHz x 1024
-# SIMDs), profiles/stage_kernels_batch_valu.md): the bound of these kernels.
+# Fraction of the VALU issue slots the batched stage kernels use
+# (rocprofv3 SQ_INSTS_VALU x 4 cycles / (kernel time x 2.4 GHz x 1024
+# SIMDs), VALU_PROFILE): the bound of these kernels.
+VALU_PROFILE = "profiles/stage_kernels_batch_valu.md"
 VALU_ISSUE_FRAC_FP32 = {
-    (8, 16, 16): {"stage0": 0.86, "stage1": 0.77, "stage2": 0.75},
+    ((8, 16, 16), "fp32_bulk"): {"stage0": 0.86, "stage1": 0.77, "stage2": 0.75},
 }
+
+
+def whole_step_roofline(cell_steps, seconds, bytes_per_cell_step):
+    """HBM fraction of the whole march: cell-steps x algorithmic bytes per
+    cell-step / seconds / peak (kernel launches, host work and idle time
+    included)."""
+    if not seconds > 0:
+        return None
+    gbs = cell_steps * bytes_per_cell_step / seconds / 1e9
+    return {"bytes_per_cell_step": bytes_per_cell_step, "achieved": round(gbs, 2),
+            "frac": round(gbs / PEAK_HBM_GBS, 5), "unit": "GB/s"}
 
 
 def workload_config(precision):
@@ -183,15 +201,19 @@ def roofline_of(stats, stage_bytes, cells_per_launch, traffic_per_cell=None, val
         "traffic": (round(traffic_per_cell[name] * cells) if traffic_per_cell else None),
         "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
         "algorithmic_bytes_per_cell": per_cell, "cells_per_launch": round(cells),
+        "traffic_profile": TRAFFIC_PROFILE if traffic_per_cell else None,
         "valu_issue_frac_profiled": (valu_frac or {}).get(name),
+        "valu_profile": VALU_PROFILE if valu_frac else None,
     }
 
 
-def batch_config(tile_threshold=None):
+def batch_config(tile_threshold=None, precision=None):
     from proton_solver.batch import BATCH_TILE_THRESHOLD, batch_workload_config
+    from proton_solver.enums import Precision
 
     return batch_workload_config(BATCH_TILE_THRESHOLD if tile_threshold is None
-                                 else tile_threshold)
+                                 else tile_threshold,
+                                 precision=Precision.FP32_BULK if precision is None else precision)
 
 
 def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_beamlet):
@@ -227,14 +249,76 @@ def batch_cpu_baseline(phantom, beamlet, box, cfg, sample_steps, cell_steps_per_
     }
 
 
-def run_batch(args):
-    from proton_solver import _native, metrics
-    from proton_solver.batch import (
-        batch_workload, beamlet_boxes, full_problem, prepare_batch,
-    )
+def plan_timed_steps(requested, batch_s, spent_s, budget_s, reserve_s, timed_cap_s):
+    """Timed batches actually run: the requested count, cut so that the timed
+    region lasts at most timed_cap_s and the whole command (spent_s so far,
+    reserve_s still needed after the timed region) stays within budget_s;
+    at least one."""
+    room = min(float(timed_cap_s), float(budget_s) - float(spent_s) - float(reserve_s))
+    fit = int(room // float(batch_s)) if batch_s > 0 else requested
+    return max(1, min(int(requested), fit))
 
-    cfg = batch_config(args.tile_threshold)
+
+def plan_extra_warmup(requested, done, batch_s, spent_s, budget_s, reserve_s, timed_cap_s):
+    """Whether one more untimed warm-up batch fits: at most two warm-ups, and
+    only while the timed region it precedes keeps its full length."""
+    if done >= min(int(requested), 2):
+        return False
+    timed = min(float(timed_cap_s), float(batch_s))
+    return float(spent_s) + float(batch_s) + timed + float(reserve_s) <= float(budget_s)
+
+
+def fp64_sample_order(energies, n):
+    """Beamlet indices of the FP64 samples in the order they are run: the
+    median energy rank first, then the lowest and the highest, then further
+    evenly spaced ranks (the budget may stop the list early)."""
+    B = len(energies)
+    order = np.argsort(energies, kind="stable")
+    ranks = [0.5, 0.0, 1.0] + [q for q in np.linspace(0.0, 1.0, max(n, 3) + 2)[1:-1]
+                                if q not in (0.5,)]
+    out = []
+    for q in ranks:
+        b = int(order[int(round(q * (B - 1)))])
+        if b not in out:
+            out.append(b)
+        if len(out) == n:
+            break
+    return out
+
+
+def fp64_batch_rate(samples, batch_steps):
+    """FP64 beamlets/s of the whole batch from the sampled FP64 marches: the
+    full-phantom march costs about the same per energy step whatever the
+    beamlet, so the batch's FP64 time is the sampled seconds per step times
+    the FP64 steps of every beamlet, estimated as its batch steps times the
+    sampled FP64 / batch step ratio."""
+    sec_per_step = float(np.mean([r["fp64_seconds"] / r["fp64_energy_steps"] for r in samples]))
+    ratio = float(np.mean([r["fp64_energy_steps"] / r["batch_energy_steps"] for r in samples]))
+    total = sec_per_step * ratio * float(np.sum(batch_steps))
+    return len(batch_steps) / total, sec_per_step, ratio
+
+
+DTYPE_LABEL = {
+    "fp32_bulk": "f32 (state, WENO-Z, stage algebra, closure with error-free 1-f^2, HLL) / "
+                 "f64 IF factors and inflow / compensated f32-pair dose",
+    "fp32_bulk_f64_closure": "f32 bulk (state, WENO-Z, stage algebra) / f64 face projection, "
+                             "closure, eigenvalues, HLL, IF factors / compensated f32-pair dose",
+}
+
+
+def run_batch(args):
+    from proton_solver import Precision, _native, metrics, tileskip
+    from proton_solver.batch import batch_workload, beamlet_boxes, full_problem, prepare_batch
+
+    precision = {"fp32_bulk": Precision.FP32_BULK,
+                 "fp32_bulk_f64_closure": Precision.FP32_BULK_F64_CLOSURE}[args.batch_precision]
+    cfg = batch_config(args.tile_threshold, precision)
     phantom, beamlets = batch_workload(n_beamlets=args.beamlets)
+    B = len(beamlets)
+
+    def spent():
+        return time.perf_counter() - T_START
+
     # Tile shape and launch from the autotuner (a sample of the beamlets,
     # outside the timed region).
     tuned = None
@@ -259,27 +343,52 @@ def run_batch(args):
         choice.apply(handle)
     setup_s = time.perf_counter() - t0
     num = _native.numerics_struct(cfg)
-    from proton_solver import tileskip
 
     use_net = args.skip_network == "on" or (args.skip_network == "auto" and
                                             tileskip.WEIGHTS_FILE.exists())
     net_opts = tileskip.SkipOptions()
     model = tileskip.load_model(0) if use_net else None
     tileskip.enable(handle, model, net_opts)
-    for _ in range(args.warmup):
-        handle.march(num, want_dose=False)
+
+    # Untimed warm-up.  The first one is the profiled march: HIP events on
+    # the march stream around every kernel (which launches every step one by
+    # one) give the kernel times and the roofline; the timed batches run
+    # without them.
     handle.set_profiling(True)
     handle.reset_stats()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    handle.march(num, want_dose=False)
+    profiled_s = time.perf_counter() - t0
+    stats = handle.kernel_stats()
+    ev_p, full_p = handle.tile_stats()
+    handle.set_profiling(False)
+    print(json.dumps({"warmup": {"profiled_batch_s": round(profiled_s, 3),
+                                 "spent_s": round(spent(), 1)}}), file=sys.stderr, flush=True)
+    batch_s = profiled_s
+    warm_done = 1
+    reserve = (1.2 * batch_s if (use_net and not args.no_network_reference) else 0.0) + \
+        (args.fp64_reserve_s if args.fp64_sample > 0 else 0.0) + \
+        (args.cpu_reserve_s if not args.no_cpu_baseline else 0.0)
+    while plan_extra_warmup(args.warmup, warm_done, batch_s, spent(), args.time_budget, reserve,
+                            args.timed_seconds):
+        t0 = time.perf_counter()
+        handle.march(num, want_dose=False)
+        batch_s = time.perf_counter() - t0
+        warm_done += 1
+    K = plan_timed_steps(args.steps, batch_s, spent(), args.time_budget, reserve,
+                         args.timed_seconds)
+    print(json.dumps({"plan": {"warmup": warm_done, "steps": K, "batch_s": round(batch_s, 3),
+                               "reserve_s": round(reserve, 1), "spent_s": round(spent(), 1)}}),
+          file=sys.stderr, flush=True)
+    handle.reset_stats()
+    host_ms0 = handle.host_ms()
+    t0 = time.perf_counter()
+    for _ in range(K):
         dose, steps = handle.march(num, want_dose=False)
     elapsed = time.perf_counter() - t0
-    stats = handle.kernel_stats()
     ev, full = handle.tile_stats()
-    host_ms = handle.host_ms()
-    handle.set_profiling(False)
-    K = max(args.steps, 1)
-    B = len(beamlets)
+    host_ms = handle.host_ms() - host_ms0
+    ev_per_batch = ev / K
 
     # The network's effect: the same batch marched with the dilation rule alone
     # (outside the timed region).
@@ -308,7 +417,7 @@ def run_batch(args):
             net.update({
                 "dilation_only_seconds": round(sec0, 3),
                 "dilation_only_beamlets_per_s": B / sec0,
-                "tiles_vs_dilation_only": ev / K / ev0,
+                "tiles_vs_dilation_only": ev_per_batch / ev0,
                 "worst_beamlet_max_dose_change_over_max": worst,
             })
         print(json.dumps({"tile_skip_network": net}), file=sys.stderr, flush=True)
@@ -322,11 +431,14 @@ def run_batch(args):
     eval_frac = ev / full if full else 1.0
 
     def cells_per_launch(launches):
-        return ev * float(tx * ty * tz) * fill / launches
+        return ev_p * float(tx * ty * tz) * fill / launches
 
     roof = roofline_of(stats, STAGE_BYTES_FP32, cells_per_launch,
-                       PMC_BYTES_PER_CELL_FP32.get(tuple(handle.tile_shape)),
-                       VALU_ISSUE_FRAC_FP32.get(tuple(handle.tile_shape)))
+                       PMC_BYTES_PER_CELL_FP32.get((tuple(handle.tile_shape), args.batch_precision)),
+                       VALU_ISSUE_FRAC_FP32.get((tuple(handle.tile_shape), args.batch_precision)))
+    if roof is not None:
+        roof["measured_in"] = "the profiled warm-up batch (untimed)"
+        roof["whole_step"] = whole_step_roofline(cell_steps * eval_frac, elapsed, STEP_BYTES_FP32)
     if args.profile_calibration:
         # a kernel of known traffic for scaling the HBM counters: the beamlet
         # dose read-out reads a float pair and writes a double per box cell
@@ -336,22 +448,31 @@ def run_batch(args):
                           "bytes_read": 8 * nx * ny * nz, "bytes_written": 8 * nx * ny * nz}),
               file=sys.stderr, flush=True)
 
-    # Plain FP64 full-domain march of a sample of the beamlets (outside the timed region).
+    # Plain FP64 full-domain march of a sample of the beamlets (outside the
+    # timed region), as many as the budget allows.
     fp64 = None
     if args.fp64_sample > 0:
-        order = np.argsort([b.energy for b in beamlets])
-        picks = [int(order[int(round(q * (B - 1)))])
-                 for q in np.linspace(0.0, 1.0, args.fp64_sample)]
+        picks = fp64_sample_order([b.energy for b in beamlets], args.fp64_sample)
         ctx = _native.Context(0)
         num64 = _native.numerics_struct(cfg, _native.FP64)
         rows = []
+        skipped = []
+        sec_per_step = args.fp64_prior_step_s
         for b in picks:
+            predicted = sec_per_step * 1.15 * float(steps[b])
+            left = args.time_budget - spent() - (args.cpu_reserve_s if not args.no_cpu_baseline
+                                                 else 0.0)
+            if rows and predicted > left:
+                skipped.append({"beamlet": b, "energy_MeV": round(beamlets[b].energy, 3),
+                                "predicted_s": round(predicted, 1)})
+                continue
             grid, mm, inflow, beam = full_problem(phantom, beamlets[b])
             ctx.set_problem(grid, mm, inflow, beam)
             t1 = time.perf_counter()
             E64, _ = ctx.march(num64)
             sec = time.perf_counter() - t1
             ref = ctx.get_dose()
+            sec_per_step = sec / (len(E64) - 1)
             x0, y0, nx, ny, nz = boxes[b]
             got = np.zeros(phantom.shape)
             got[x0:x0 + nx, y0:y0 + ny, :nz] = beamlet_dose(b)
@@ -367,18 +488,25 @@ def run_batch(args):
                 "max_dose_rel_err": metrics.maximum_dose_relative_difference(got, ref),
                 "max_abs_diff_over_max": metrics.max_abs_difference_over_max(got, ref),
                 "gamma_1pct_1mm_pass_percent": metrics.gamma_pass_rate(
-                    got[sl], ref[sl], phantom.spacing, 1.0, 1.0, 10.0),
+                    np.ascontiguousarray(got[sl]), np.ascontiguousarray(ref[sl]),
+                    phantom.spacing, 1.0, 1.0, 10.0, device=0),
                 "fp64_dose_outside_gamma_region_over_max": float(outside.max() / ref.max()),
             })
             print(json.dumps({"fp64_sample": rows[-1]}), file=sys.stderr, flush=True)
-        mean_s = float(np.mean([r["fp64_seconds"] for r in rows]))
+        rate, sps, ratio = fp64_batch_rate(rows, steps)
         fp64 = {
             "reference": "plain FP64 march of the beamlet on the whole 200x200x300 phantom "
                          "(bitwise the CPU oracle)",
             "samples": rows,
-            "fp64_beamlets_per_s": 1.0 / mean_s,
-            "note": "beamlets at evenly spaced energy ranks of the batch, lowest to highest; "
-                    "the FP64 beamlets/s is the inverse of the sample's mean time",
+            "skipped_for_time": skipped,
+            "fp64_seconds_per_energy_step": sps,
+            "fp64_over_batch_energy_steps": ratio,
+            "fp64_beamlets_per_s": rate,
+            "fp64_beamlets_per_s_sample_mean": 1.0 / float(np.mean([r["fp64_seconds"]
+                                                                     for r in rows])),
+            "note": "fp64_beamlets_per_s is the whole batch at the sampled FP64 seconds per "
+                    "energy step and FP64/batch step ratio; the samples are the median, "
+                    "lowest and highest energy ranks, in that order, as the time budget allows",
             "worst_max_dose_rel_err": max(abs(r["max_dose_rel_err"]) for r in rows),
             "min_gamma_pass_percent": min(r["gamma_1pct_1mm_pass_percent"] for r in rows),
         }
@@ -394,13 +522,13 @@ def run_batch(args):
         "value": value,
         "unit": "beamlet dose solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": warm_done,
         "ms_per_step": 1000.0 * elapsed / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 bulk / f64 closure+projection+dose",
+        "dtype": DTYPE_LABEL[args.batch_precision],
         "data": "synthetic seeded phantom and beamlets (no external data)",
         "config": {
             "workload": "256 pencil beamlets in one batch, 200x200x300 @1 mm water phantom with "
@@ -408,12 +536,15 @@ def run_batch(args):
                         "FP32 bulk with tile skipping" +
                         (" and the tile-skip network" if use_net else ""),
             "n_beamlets": B,
+            "precision": args.batch_precision,
             "tile_shape": list(handle.tile_shape),
             "launch": tuned["mode"] if tuned else "per_tile",
             "box_cells_total": int(box_cells.sum()),
             "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
                                          "max": int(steps.max())},
             "tile_threshold": cfg.device.tile_threshold,
+            "requested_steps": args.steps, "requested_warmup": args.warmup,
+            "time_budget_s": args.time_budget,
         },
         "cell_steps_per_s": cell_steps / elapsed,
         "cell_steps_per_s_evaluated": cell_steps * eval_frac / elapsed,
@@ -423,11 +554,13 @@ def run_batch(args):
         "autotune": tuned,
         "host_step_setup_ms_per_batch": host_ms / K,
         "setup_s": setup_s,
-        "kernel_ms_per_step": {k: round(v[1] / K, 3) for k, v in stats.items() if v[0]},
+        "kernel_ms_per_batch_profiled": {k: round(v[1], 3) for k, v in stats.items() if v[0]},
+        "profiled_batch_s": profiled_s,
         "fp64_full_domain": fp64,
         "speedup_vs_fp64_full_domain": (value / fp64["fp64_beamlets_per_s"]) if fp64 else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "wall_s": round(spent(), 1),
     }
     print(json.dumps(line), flush=True)
 
@@ -567,6 +700,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--beamlets", type=int, default=256)
     ap.add_argument("--fp64-sample", type=int, default=3)
+    ap.add_argument("--batch-precision", default="fp32_bulk",
+                    choices=["fp32_bulk", "fp32_bulk_f64_closure"])
+    ap.add_argument("--time-budget", type=float, default=450.0,
+                    help="wall-clock seconds the whole command should stay within; the timed "
+                         "and warm-up batches are cut to fit (the counts run are reported)")
+    ap.add_argument("--timed-seconds", type=float, default=60.0,
+                    help="upper bound of the timed region, seconds")
+    ap.add_argument("--fp64-reserve-s", type=float, default=60.0,
+                    help="seconds kept for the FP64 samples when planning the timed batches")
+    ap.add_argument("--fp64-prior-step-s", type=float, default=0.03,
+                    help="FP64 seconds per energy step assumed before the first sample")
+    ap.add_argument("--cpu-reserve-s", type=float, default=30.0,
+                    help="seconds kept for the CPU baseline")
     ap.add_argument("--tile-threshold", type=float, default=None,
                     help="tile threshold relative to the peak inflow fluence "
                          "(default: proton_solver.batch.BATCH_TILE_THRESHOLD)")
@@ -576,7 +722,7 @@ def main():
                     help="skip the untimed march without the network")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="default tile shape and launch instead of the autotuner's choice")
-    ap.add_argument("--autotune-sample", type=int, default=32)
+    ap.add_argument("--autotune-sample", type=int, default=24)
     ap.add_argument("--tile-shape", type=int, default=None,
                     help="index of the batched kernels' tile shape (no autotuning)")
     ap.add_argument("--profile-calibration", action="store_true",
@@ -591,7 +737,7 @@ def main():
         sys.exit(2)
 
     if args.workload == "batch256":
-        args.steps = 1 if args.steps is None else args.steps
+        args.steps = 3 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         # about 10 s of oracle work on the host's cores
         args.cpu_sample_steps = 100 if args.cpu_sample_steps is None else args.cpu_sample_steps
@@ -640,11 +786,17 @@ def run_single(args):
         fp64 = {"seconds": time.perf_counter() - t0, "steps": len(E64) - 1,
                 "dose": ctx.get_dose()}
 
-    for _ in range(args.warmup):
-        ctx.set_state(None)
-        ctx.march(num)
+    # Untimed warm-up; the first warm-up march is the profiled one (HIP events
+    # around every kernel), the timed marches run without the events.
     ctx.set_profiling(True)
     ctx.reset_stats()
+    for w in range(max(args.warmup, 1)):
+        ctx.set_state(None)
+        ctx.march(num)
+        if w == 0:
+            stats = ctx.kernel_stats()
+            tiles_eval, tiles_full = ctx.tile_stats()
+            ctx.set_profiling(False)
     steps_per_march = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -652,9 +804,6 @@ def run_single(args):
         E, _ = ctx.march(num)
         steps_per_march.append(len(E) - 1)
     elapsed = time.perf_counter() - t0
-    stats = ctx.kernel_stats()
-    tiles_eval, tiles_full = ctx.tile_stats()
-    ctx.set_profiling(False)
     dose = ctx.get_dose()
 
     n_energy_steps = steps_per_march[-1]
@@ -662,8 +811,9 @@ def run_single(args):
     solves_per_s = args.steps / elapsed
     cell_steps = n_cells * sum(steps_per_march)
 
-    # Dominant stage kernel and its HBM roofline.  Cells per launch are the
-    # cells of the evaluated tiles (all cells when no tile is skipped).
+    # Dominant stage kernel and its HBM roofline (from the profiled warm-up
+    # march).  Cells per launch are the cells of the evaluated tiles (all
+    # cells when no tile is skipped).
     eval_frac = (tiles_eval / tiles_full) if tiles_full else 1.0
     cells_per_launch = n_cells * eval_frac
     best = None
@@ -684,6 +834,10 @@ def run_single(args):
             "kernel": name, "avg_launch_ms": round(avg_s * 1000, 4),
             "algorithmic_bytes_per_cell": per_cell,
             "cells_per_launch": round(cells_per_launch),
+            "measured_in": "the profiled warm-up march (untimed)",
+            "whole_step": whole_step_roofline(
+                cell_steps * eval_frac, elapsed,
+                STEP_BYTES_FP32 if precision is Precision.FP32_BULK else STEP_BYTES_FP64),
         }
 
     accuracy = None

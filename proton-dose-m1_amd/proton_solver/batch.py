@@ -186,13 +186,14 @@ def air_free_beamlets(phantom: Phantom, beamlets, boxes, margin_cells=48):
 BATCH_TILE_THRESHOLD = 1e-6
 
 
-def batch_workload_config(tile_threshold=BATCH_TILE_THRESHOLD, vacuum_aware_cfl=False):
+def batch_workload_config(tile_threshold=BATCH_TILE_THRESHOLD, vacuum_aware_cfl=False,
+                          precision=Precision.FP32_BULK):
     """Physics and numerics of the batched workloads: Wentzel scattering and
     nuclear removal, WENO-Z, IF-SSPRK3 with exact M1 speeds, max step
     0.25 MeV, CFL 0.9 over the whole phantom (as the march of one beamlet on
     the phantom takes it; vacuum_aware_cfl selects the reference's
     vacuum-aware rule, energy.py:459-537, instead), energy_min 1 MeV,
-    FP32-bulk with tile skipping at BATCH_TILE_THRESHOLD.  Its geometry and
+    FP32-bulk (or `precision`) with tile skipping at BATCH_TILE_THRESHOLD.  Its geometry and
     beam are placeholders: the batch takes the phantom and the beamlets."""
     from .beam import Beam as _Beam
     from .config import DeviceOptions, Numerics, Physics, SolverConfig
@@ -206,7 +207,7 @@ def batch_workload_config(tile_threshold=BATCH_TILE_THRESHOLD, vacuum_aware_cfl=
                           energy_scheme=EnergyScheme.IF_SSPRK3, wave_speeds=WaveSpeeds.EXACT,
                           max_energy_step=0.25, cfl=0.9, energy_min=1.0,
                           vacuum_aware_cfl=vacuum_aware_cfl),
-        device=DeviceOptions(precision=Precision.FP32_BULK, tile_threshold=tile_threshold),
+        device=DeviceOptions(precision=precision, tile_threshold=tile_threshold),
     )
 
 
