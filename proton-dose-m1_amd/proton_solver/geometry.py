@@ -1,7 +1,12 @@
-"""Domain geometry (mirrors ``proton_solver/geometry.py:10-271``).
+"""Domain geometry: the ``proton_solver.geometry`` interface
+(reference ``src/proton_solver/geometry.py:10-271``).
 
-Box geometries, CT scans and the HU partition keep the reference's fields,
-half-open conventions and validation messages.
+The public classes keep the reference's fields, defaults, half-open
+conventions, exception types and messages, since callers construct them
+directly.  The implementation is this package's own: validation goes
+through a few shared predicates, a CT file is read by ``read_ct_npz`` and
+oriented by ``orient_volume``, and box overlaps are found with a sweep over
+the boxes sorted along x.
 """
 
 import math
@@ -12,10 +17,34 @@ import numpy as np
 
 from .enums import Material
 
+# Array names a CT NPZ may use for its HU volume, in order of preference; a
+# file holding one unnamed array (np.savez(path, array)) stores it as arr_0.
+_HU_KEYS = ("HU", "hu")
+_UNNAMED_KEY = "arr_0"
+
+
+def _require(ok: bool, message: str):
+    if not ok:
+        raise ValueError(message)
+
+
+def _all_finite(values) -> bool:
+    return all(math.isfinite(v) for v in values)
+
+
+def _positive_finite_triple(values) -> bool:
+    return len(values) == 3 and all(math.isfinite(v) and v > 0 for v in values)
+
+
+def _check_hu_volume(hu: np.ndarray):
+    _require(hu.ndim == 3 and np.issubdtype(hu.dtype, np.number),
+             "CT HU must be a numeric 3D array.")
+    _require(bool(np.isfinite(hu).all()), "CT HU contains non-finite values.")
+
 
 @dataclass(frozen=True)
 class Box:
-    """Axis-aligned box using half-open intervals [min, max)."""
+    """Axis-aligned box using half-open intervals [min, max), in metres."""
 
     x_min: float
     x_max: float
@@ -27,18 +56,28 @@ class Box:
     material: Material
 
     def __post_init__(self):
-        coords = (self.x_min, self.x_max, self.y_min, self.y_max,
-                  self.z_min, self.z_max)
-        if not all(math.isfinite(v) for v in coords):
-            raise ValueError("Box coordinates must be finite.")
-        if (self.x_min >= self.x_max or self.y_min >= self.y_max
-                or self.z_min >= self.z_max):
-            raise ValueError("Box minima must be smaller than maxima.")
+        _require(_all_finite(self.bounds()), "Box coordinates must be finite.")
+        _require(all(lo < hi for lo, hi in self.intervals()),
+                 "Box minima must be smaller than maxima.")
+
+    def bounds(self):
+        return (self.x_min, self.x_max, self.y_min, self.y_max, self.z_min, self.z_max)
+
+    def intervals(self):
+        """((x_min, x_max), (y_min, y_max), (z_min, z_max))."""
+        b = self.bounds()
+        return ((b[0], b[1]), (b[2], b[3]), (b[4], b[5]))
+
+    def overlaps(self, other: "Box") -> bool:
+        """Two half-open boxes share a point when they overlap on every axis."""
+        return all(a_lo < b_hi and b_lo < a_hi
+                   for (a_lo, a_hi), (b_lo, b_hi) in zip(self.intervals(), other.intervals()))
 
 
 @dataclass(frozen=True)
 class HUPartition:
-    """Map HU values to transport materials with half-open HU bins."""
+    """Half-open HU bins mapped to transport materials: bin i holds
+    cutoffs[i-1] <= HU < cutoffs[i] and is transported as materials[i]."""
 
     cutoffs: tuple[float, ...] = (-200.0, 150.0)
     materials: tuple[Material, ...] = (
@@ -48,28 +87,50 @@ class HUPartition:
     )
 
     def __post_init__(self):
-        if len(self.materials) != len(self.cutoffs) + 1:
-            raise ValueError("HUPartition needs one more material than cutoffs.")
-        if not all(isinstance(m, Material) for m in self.materials):
-            raise ValueError("HUPartition materials must be Material values.")
-        if not all(math.isfinite(c) for c in self.cutoffs):
-            raise ValueError("HU cutoffs must be finite.")
-        if any(a >= b for a, b in zip(self.cutoffs, self.cutoffs[1:])):
-            raise ValueError("HU cutoffs must be strictly increasing.")
+        _require(len(self.materials) == len(self.cutoffs) + 1,
+                 "HUPartition needs one more material than cutoffs.")
+        _require(all(isinstance(m, Material) for m in self.materials),
+                 "HUPartition materials must be Material values.")
+        _require(_all_finite(self.cutoffs), "HU cutoffs must be finite.")
+        _require(all(np.diff(np.asarray(self.cutoffs, dtype=float)) > 0),
+                 "HU cutoffs must be strictly increasing.")
 
     def classify(self, hu: np.ndarray) -> np.ndarray:
-        """Integer class of every voxel (geometry.py:63-70)."""
+        """Bin index of every voxel (int32): the number of cutoffs <= HU."""
         hu = np.asarray(hu)
-        if hu.ndim != 3 or not np.issubdtype(hu.dtype, np.number):
-            raise ValueError("CT HU must be a numeric 3D array.")
-        if not np.all(np.isfinite(hu)):
-            raise ValueError("CT HU contains non-finite values.")
+        _check_hu_volume(hu)
         return np.searchsorted(self.cutoffs, hu, side="right").astype(np.int32)
+
+
+def read_ct_npz(path):
+    """(HU volume, body mask or None) stored in a CT NPZ, unpickled never."""
+    with np.load(path, allow_pickle=False) as archive:
+        names = archive.files
+        key = next((k for k in _HU_KEYS if k in names), None)
+        if key is None and names == [_UNNAMED_KEY]:
+            key = _UNNAMED_KEY
+        _require(key is not None, "CT NPZ needs a 'HU', 'hu', or single 'arr_0' array.")
+        hu = np.asarray(archive[key])
+        mask = np.asarray(archive["body_mask"]) if "body_mask" in names else None
+    return hu, mask
+
+
+def orient_volume(volume: np.ndarray, axis_order, flip_axes) -> np.ndarray:
+    """Solver-oriented copy of a volume: its axes permuted to axis_order,
+    then reversed along every solver axis flagged in flip_axes."""
+    out = np.transpose(volume, tuple(axis_order))
+    flipped = tuple(a for a, f in enumerate(flip_axes) if f)
+    if flipped:
+        out = np.flip(out, axis=flipped)
+    return np.ascontiguousarray(out)
 
 
 @dataclass(frozen=True)
 class CTScan:
-    """A CT volume oriented as solver (x, y, +z) (geometry.py:73-135)."""
+    """A CT volume read from an NPZ (``HU``, ``hu`` or a single ``arr_0``
+    array, optional ``body_mask``) and oriented as solver (x, y, +z).
+    source_spacing_m gives the voxel size of the stored axes, since the file
+    need not hold it."""
 
     path: str | Path
     source_spacing_m: tuple[float, float, float]
@@ -81,56 +142,46 @@ class CTScan:
     spacing: tuple[float, float, float] = field(init=False)
 
     def __post_init__(self):
-        if len(self.source_spacing_m) != 3 or not all(
-            math.isfinite(s) and s > 0 for s in self.source_spacing_m
-        ):
-            raise ValueError("CT spacing must contain three positive finite metres.")
-        if tuple(sorted(self.axis_order)) != (0, 1, 2):
-            raise ValueError("axis_order must be a permutation of (0, 1, 2).")
-        if len(self.flip_axes) != 3 or any(type(f) is not bool for f in self.flip_axes):
-            raise ValueError("flip_axes must contain three booleans.")
+        _require(_positive_finite_triple(self.source_spacing_m),
+                 "CT spacing must contain three positive finite metres.")
+        _require(sorted(self.axis_order) == [0, 1, 2],
+                 "axis_order must be a permutation of (0, 1, 2).")
+        _require(len(self.flip_axes) == 3 and all(type(f) is bool for f in self.flip_axes),
+                 "flip_axes must contain three booleans.")
 
-        with np.load(self.path, allow_pickle=False) as data:
-            if "HU" in data:
-                key = "HU"
-            elif "hu" in data:
-                key = "hu"
-            elif data.files == ["arr_0"]:
-                key = "arr_0"
-            else:
-                raise ValueError("CT NPZ needs a 'HU', 'hu', or single 'arr_0' array.")
-            hu = np.asarray(data[key])
-            mask = np.asarray(data["body_mask"]) if "body_mask" in data else None
+        hu, mask = read_ct_npz(self.path)
+        _check_hu_volume(hu)
+        _require(mask is None or mask.shape == hu.shape,
+                 "body_mask shape must match the HU array.")
 
-        if hu.ndim != 3 or not np.issubdtype(hu.dtype, np.number):
-            raise ValueError("CT HU must be a numeric 3D array.")
-        if not np.all(np.isfinite(hu)):
-            raise ValueError("CT HU contains non-finite values.")
-        if mask is not None and mask.shape != hu.shape:
-            raise ValueError("body_mask shape must match the HU array.")
-
-        hu = np.transpose(hu, self.axis_order)
-        if mask is not None:
-            mask = np.transpose(mask.astype(bool), self.axis_order)
-        for axis, flip in enumerate(self.flip_axes):
-            if flip:
-                hu = np.flip(hu, axis=axis)
-                if mask is not None:
-                    mask = np.flip(mask, axis=axis)
-
-        object.__setattr__(self, "hu", np.ascontiguousarray(hu))
-        object.__setattr__(
-            self, "body_mask",
-            None if mask is None else np.ascontiguousarray(mask),
-        )
-        object.__setattr__(
-            self, "spacing",
-            tuple(self.source_spacing_m[i] for i in self.axis_order),
-        )
+        oriented = {
+            "hu": orient_volume(hu, self.axis_order, self.flip_axes),
+            "body_mask": (None if mask is None else
+                          orient_volume(mask.astype(bool), self.axis_order, self.flip_axes)),
+            "spacing": tuple(self.source_spacing_m[a] for a in self.axis_order),
+        }
+        for name, value in oriented.items():
+            object.__setattr__(self, name, value)
 
     @property
     def shape(self) -> tuple[int, int, int]:
         return self.hu.shape
+
+
+def _first_overlap(boxes):
+    """A pair of overlapping boxes, or None: boxes sorted by x_min are swept
+    along x, each compared only with the earlier boxes whose x interval is
+    still open."""
+    order = sorted(range(len(boxes)), key=lambda i: boxes[i].x_min)
+    open_boxes = []
+    for i in order:
+        b = boxes[i]
+        open_boxes = [a for a in open_boxes if a.x_max > b.x_min]
+        for a in open_boxes:
+            if a.overlaps(b):
+                return a, b
+        open_boxes.append(b)
+    return None
 
 
 @dataclass
@@ -147,27 +198,37 @@ class Geometry:
     hu_partition: HUPartition = field(default_factory=HUPartition)
 
     def __post_init__(self):
-        if len(self.size) != 3 or len(self.resolution) != 3:
-            raise ValueError("size and resolution must contain exactly three values.")
-        if not all(math.isfinite(x) and x > 0 for x in self.size):
-            raise ValueError("Geometry dimensions must be finite and positive.")
-        if not all(type(n) is int and n > 0 for n in self.resolution):
-            raise ValueError("Geometry resolution must contain positive integers.")
-
+        _require(len(self.size) == 3 and len(self.resolution) == 3,
+                 "size and resolution must contain exactly three values.")
+        _require(_positive_finite_triple(self.size),
+                 "Geometry dimensions must be finite and positive.")
+        _require(all(type(n) is int and n > 0 for n in self.resolution),
+                 "Geometry resolution must contain positive integers.")
         self._verify_objects()
-
         if self.ct_scan is not None:
-            if self.objects:
-                raise ValueError("CT and Box objects cannot be combined.")
-            if self.resolution != self.ct_scan.shape:
-                raise ValueError("Geometry resolution must match the CT array shape.")
-            if not all(math.isclose(a, b, rel_tol=1e-9, abs_tol=1e-12)
-                       for a, b in zip(self.spacing, self.ct_scan.spacing)):
-                raise ValueError("Geometry cell spacing must match the CT spacing.")
-            if self.view_geometry:
-                raise ValueError(
-                    "3D Box viewer does not support CT; set view_geometry=False."
-                )
+            self._verify_ct()
+
+    def _verify_objects(self):
+        """Every box inside the domain (x, y centred on 0, z from 0), no two
+        overlapping."""
+        half_x, half_y = self.size[0] / 2, self.size[1] / 2
+        domain = ((-half_x, half_x), (-half_y, half_y), (0.0, self.size[2]))
+        for box in self.objects:
+            inside = all(lo <= b_lo < b_hi <= hi
+                         for (b_lo, b_hi), (lo, hi) in zip(box.intervals(), domain))
+            _require(inside, "Geometry object extends outside the domain.")
+        _require(_first_overlap(self.objects) is None, "Geometry objects may not overlap.")
+
+    def _verify_ct(self):
+        scan = self.ct_scan
+        _require(not self.objects, "CT and Box objects cannot be combined.")
+        _require(self.resolution == scan.shape,
+                 "Geometry resolution must match the CT array shape.")
+        _require(all(math.isclose(a, b, rel_tol=1e-9, abs_tol=1e-12)
+                     for a, b in zip(self.spacing, scan.spacing)),
+                 "Geometry cell spacing must match the CT spacing.")
+        _require(not self.view_geometry,
+                 "3D Box viewer does not support CT; set view_geometry=False.")
 
     @classmethod
     def from_ct_scan(
@@ -180,52 +241,53 @@ class Geometry:
         hu_partition=None,
         background_material=Material.WATER,
     ) -> "Geometry":
-        """Domain that matches an oriented CT voxel grid (geometry.py:160-185)."""
-        scan = CTScan(
-            path=path,
-            source_spacing_m=source_spacing_m,
-            axis_order=axis_order,
-            flip_axes=flip_axes,
-        )
-        return cls(
-            size=tuple(n * d for n, d in zip(scan.shape, scan.spacing)),
-            resolution=scan.shape,
-            background_material=background_material,
-            ct_scan=scan,
-            hu_partition=hu_partition if hu_partition is not None else HUPartition(),
-        )
+        """The domain of an oriented CT volume: one cell per voxel."""
+        scan = CTScan(path=path, source_spacing_m=source_spacing_m, axis_order=axis_order,
+                      flip_axes=flip_axes)
+        extent = tuple(n * d for n, d in zip(scan.shape, scan.spacing))
+        return cls(size=extent, resolution=scan.shape, background_material=background_material,
+                   ct_scan=scan,
+                   hu_partition=HUPartition() if hu_partition is None else hu_partition)
 
     @staticmethod
     def _boxes_overlap(a: Box, b: Box) -> bool:
-        return (
-            a.x_min < b.x_max and b.x_min < a.x_max
-            and a.y_min < b.y_max and b.y_min < a.y_max
-            and a.z_min < b.z_max and b.z_min < a.z_max
-        )
+        return a.overlaps(b)
 
-    def _verify_objects(self):
-        lx, ly, lz = self.size
-        for obj in self.objects:
-            if not (
-                -lx / 2 <= obj.x_min < obj.x_max <= lx / 2
-                and -ly / 2 <= obj.y_min < obj.y_max <= ly / 2
-                and 0.0 <= obj.z_min < obj.z_max <= lz
-            ):
-                raise ValueError("Geometry object extends outside the domain.")
-        for i, a in enumerate(self.objects):
-            for b in self.objects[i + 1:]:
-                if self._boxes_overlap(a, b):
-                    raise ValueError("Geometry objects may not overlap.")
+    @property
+    def nx(self):
+        return self.resolution[0]
 
-    nx = property(lambda self: self.resolution[0])
-    ny = property(lambda self: self.resolution[1])
-    nz = property(lambda self: self.resolution[2])
-    lx = property(lambda self: self.size[0])
-    ly = property(lambda self: self.size[1])
-    lz = property(lambda self: self.size[2])
-    dx = property(lambda self: self.lx / self.nx)
-    dy = property(lambda self: self.ly / self.ny)
-    dz = property(lambda self: self.lz / self.nz)
+    @property
+    def ny(self):
+        return self.resolution[1]
+
+    @property
+    def nz(self):
+        return self.resolution[2]
+
+    @property
+    def lx(self):
+        return self.size[0]
+
+    @property
+    def ly(self):
+        return self.size[1]
+
+    @property
+    def lz(self):
+        return self.size[2]
+
+    @property
+    def dx(self):
+        return self.lx / self.nx
+
+    @property
+    def dy(self):
+        return self.ly / self.ny
+
+    @property
+    def dz(self):
+        return self.lz / self.nz
 
     @property
     def spacing(self):
