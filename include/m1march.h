@@ -226,6 +226,27 @@ int m1_gamma_pass_count(int32_t device, const double *dose, const double *refere
                         int32_t n_offsets, double dose_tolerance, int64_t *n_pass);
 const char *m1_gamma_last_error(void);
 
+/* No counterpart in the reference: m1_gamma_pass_count for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  dose, reference (nx, ny, nz)
+ * and points (n_points x 3) are DEVICE pointers the caller owns; offsets
+ * and dist_terms are host arrays as in m1_gamma_pass_count and are staged,
+ * with the pass counter, in a fixed device array of the library (about
+ * 512 KiB per GPU, held until the process ends), so the call allocates
+ * nothing.  n_offsets above M1_GAMMA_NOALLOC_MAX_OFFSETS is rejected
+ * (M1_ERR_VALUE); points outside the grid are skipped by the kernel and
+ * reported as M1_ERR_VALUE.  The work is queued on `stream` (a hipStream_t;
+ * NULL: the default stream), which the call synchronizes before returning
+ * the count.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (several host threads, or streams whose calls are in flight at
+ * once) race and must be serialized by the caller. */
+#define M1_GAMMA_NOALLOC_MAX_OFFSETS 16384
+int m1_gamma_pass_count_noalloc(int32_t device, const double *dose, const double *reference,
+                                int32_t nx, int32_t ny, int32_t nz, const int32_t *points,
+                                int32_t n_points, const double *offsets,
+                                const double *dist_terms, int32_t n_offsets,
+                                double dose_tolerance, void *stream, int64_t *n_pass);
+
 /* spatial_operator (spatial.py:104-252, active_domain=False) of the current
  * device state at `energy`: rhs = -div F_HLL, (4,nx,ny,nz).  boundary_state
  * is the z-min inflow state (4,nx,ny) or NULL for inflow.state(energy). */

@@ -89,6 +89,10 @@ SIGNATURES = [
     ("m1_gamma_pass_count", C.c_int, [C.c_int32, _dp, _dp, C.c_int32, C.c_int32, C.c_int32, _ip,
                                       C.c_int32, _dp, _dp, C.c_int32, C.c_double,
                                       C.POINTER(C.c_int64)]),
+    ("m1_gamma_pass_count_noalloc", C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                              C.c_int32, C.c_int32, C.c_void_p, C.c_int32, _dp,
+                                              _dp, C.c_int32, C.c_double, C.c_void_p,
+                                              C.POINTER(C.c_int64)]),
     ("m1_gamma_last_error", C.c_char_p, []),
     ("m1_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_int32, C.c_double,
                           C.c_double, _dp]),
@@ -206,6 +210,32 @@ def gamma_pass_count(device, dose, reference, points, offsets, dist_terms, dose_
     rc = lib.m1_gamma_pass_count(int(device), _ptr(dose), _ptr(reference), nx, ny, nz, _ptr(pts),
                                  len(pts), _ptr(offs), _ptr(dt), len(dt), float(dose_tolerance),
                                  C.byref(n))
+    if rc != M1_OK:
+        msg = lib.m1_gamma_last_error().decode()
+        raise (ValueError if rc == M1_ERR_VALUE else MarchError)(msg)
+    return int(n.value)
+
+
+GAMMA_NOALLOC_MAX_OFFSETS = 16384  # M1_GAMMA_NOALLOC_MAX_OFFSETS
+
+
+def gamma_pass_count_noalloc(device, dose_ptr, reference_ptr, shape, points_ptr, n_points,
+                             offsets, dist_terms, dose_tolerance, stream=0):
+    """m1_gamma_pass_count_noalloc: the pass count of device-resident dose,
+    reference ((nx, ny, nz) float64) and points ((n_points, 3) int32) given
+    by their device addresses; offsets and dist_terms are host arrays, staged
+    in the library's fixed device scratch (nothing is allocated).  stream is a
+    hipStream_t handle (0: the default stream)."""
+    lib = load_library()
+    offs = np.ascontiguousarray(offsets, dtype=np.float64)
+    dt = np.ascontiguousarray(dist_terms, dtype=np.float64)
+    nx, ny, nz = (int(v) for v in shape)
+    n = C.c_int64(0)
+    rc = lib.m1_gamma_pass_count_noalloc(int(device), C.c_void_p(int(dose_ptr)),
+                                         C.c_void_p(int(reference_ptr)), nx, ny, nz,
+                                         C.c_void_p(int(points_ptr)), int(n_points), _ptr(offs),
+                                         _ptr(dt), len(dt), float(dose_tolerance),
+                                         C.c_void_p(int(stream)), C.byref(n))
     if rc != M1_OK:
         msg = lib.m1_gamma_last_error().decode()
         raise (ValueError if rc == M1_ERR_VALUE else MarchError)(msg)
