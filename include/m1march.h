@@ -213,8 +213,10 @@ int m1_compute_energy_step_vacuum(m1_ctx *ctx, const m1_numerics *num, double en
  * triplets, n_points of them) whose global gamma^2 is <= 1: for each point p,
  * over the search offsets (offsets: n_offsets voxel-unit triplets, sorted so
  * that dist_terms = (length / distance)^2 is non-decreasing), the evaluated
- * dose is trilinearly interpolated at p + offset (clipped to the grid) and
- * gamma^2 = min ((D - D_ref(p)) / dose_tolerance)^2 + dist_term.  dose and
+ * dose is trilinearly interpolated at p + offset and gamma^2 = min
+ * ((D - D_ref(p)) / dose_tolerance)^2 + dist_term over the offsets whose
+ * p + offset lies inside the grid (pymedphys evaluates the dose outside the
+ * grid as +inf, so such search points never match).  dose and
  * reference are host (nx, ny, nz) arrays; device memory is allocated and
  * freed inside the call.  The arithmetic is that of
  * proton_solver.metrics.gamma_pass_rate, so the count equals the host

@@ -8,8 +8,9 @@
 // Per evaluated voxel p (reference dose >= the cutoff): for the search
 // offsets o_k sorted by length r_k (in voxels: o_k / spacing), while
 // (r_k / distance)^2 < gamma^2, the evaluated dose is trilinearly
-// interpolated at p + o_k (clipped to the grid) and
-// gamma^2 = min(gamma^2, ((D - D_ref) / dd)^2 + (r_k / distance)^2).
+// interpolated at p + o_k and gamma^2 = min(gamma^2, ((D - D_ref) / dd)^2 +
+// (r_k / distance)^2); search points outside the grid are skipped (pymedphys
+// fills the evaluated dose with +inf there).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -55,8 +56,10 @@ gamma_kernel(const double* __restrict__ dose, const double* __restrict__ ref, in
                 break;  // offsets are sorted by length: no later one can lower gamma
             int i0[3];
             double f[3];
+            bool inside = true;
             for (int a = 0; a < 3; ++a) {
                 double p = double(base[a]) + offs[3 * k + a];
+                inside = inside && p >= 0.0 && p <= hi[a];
                 p = p < 0.0 ? 0.0 : p;
                 p = p > hi[a] ? hi[a] : p;
                 int i = int(floor(p));
@@ -81,7 +84,8 @@ gamma_kernel(const double* __restrict__ dose, const double* __restrict__ ref, in
             }
             const double q = (val - rv) / dd;
             const double g = q * q + dt;
-            g2 = g < g2 ? g : g2;
+            if (inside)  // a search point outside the grid never matches
+                g2 = g < g2 ? g : g2;
         }
         pass = g2 <= 1.0;
     }

@@ -134,7 +134,8 @@ def gamma_pass_rate(dose, reference, spacing_m, dose_percent=1.0, distance_mm=1.
     of the reference maximum.  For each, the evaluated dose is searched on a
     grid of offsets with step ``distance/interp_fraction`` inside a sphere of
     radius ``max_gamma * distance``; the evaluated dose between voxel centres is
-    trilinearly interpolated.  gamma = min over offsets of
+    trilinearly interpolated; search points outside the grid do not match.
+    gamma = min over offsets of
     sqrt((dD / (dose_percent% * max_ref))^2 + (r / distance)^2); a point passes
     when gamma <= 1.
 
@@ -160,6 +161,9 @@ def gamma_pass_rate(dose, reference, spacing_m, dose_percent=1.0, distance_mm=1.
         if not np.any(alive):
             break
         p = pts[alive].astype(float) + scaled[k]
+        # a search point outside the grid never matches (pymedphys
+        # interpolates the evaluated dose with an infinite fill value there)
+        inside = np.all((p >= 0.0) & (p <= shape - 1.0), axis=1)
         p = np.clip(p, 0.0, shape - 1.0)
         i0 = np.floor(p).astype(int)
         i0 = np.minimum(i0, shape - 2).clip(min=0)
@@ -175,7 +179,7 @@ def gamma_pass_rate(dose, reference, spacing_m, dose_percent=1.0, distance_mm=1.
                     wz = f[:, 2] if dz_ else 1.0 - f[:, 2]
                     iz = np.minimum(i0[:, 2] + dz_, shape[2] - 1)
                     val += wx * wy * wz * dose[ix, iy, iz]
-        g2 = ((val - ref_vals[alive]) / dd) ** 2 + dist_term
+        g2 = np.where(inside, ((val - ref_vals[alive]) / dd) ** 2 + dist_term, np.inf)
         cur = gamma2[alive]
         gamma2[alive] = np.minimum(cur, g2)
     return 100.0 * float(np.mean(gamma2 <= 1.0))

@@ -170,3 +170,32 @@ def test_gamma_brute_force_point():
     assert g == 100.0
     g = metrics.gamma_pass_rate(ev, ref, (1e-3,) * 3, 1.0, 0.8, lower_percent_cutoff=90.0)
     assert g == 0.0
+
+
+def test_gamma_search_points_outside_the_grid_never_match():
+    """pymedphys interpolates the evaluated dose with +inf outside its grid,
+    so search points beyond the grid never match.  The host search equals a
+    brute-force search with scipy's RegularGridInterpolator filled with +inf
+    outside the grid on a case with every voxel on the grid's surface and
+    cells (1.5 mm) that the search step (0.1 mm) does not divide."""
+    ref = np.ones((3, 3, 3))
+    ev = np.full((3, 3, 3), 1.02)
+    ev[1] = 1.0
+    sp = (1.5e-3, 1.5e-3, 1.5e-3)
+    got = metrics.gamma_pass_rate(ev, ref, sp, 1.0, 1.0, 0.0)
+    # brute force over the same offsets, skipping search points outside the grid
+    setup = metrics._gamma_setup(ev, ref, sp, 1.0, 1.0, 0.0, 10, 1.1)
+    _, _, pts, scaled, dist_terms, dd = setup
+    from scipy.interpolate import RegularGridInterpolator
+
+    axes = [np.arange(n, dtype=float) for n in ev.shape]
+    interp = RegularGridInterpolator(axes, ev, bounds_error=False, fill_value=np.inf)
+    n_pass = 0
+    for p in pts:
+        q = p[None, :] + scaled
+        vals = interp(q)
+        g2 = ((vals - ref[tuple(p)]) / dd) ** 2 + dist_terms
+        n_pass += bool(np.min(g2) <= 1.0)
+    assert got == pytest.approx(100.0 * n_pass / len(pts), abs=1e-12)
+    # the exact plane passes; the others lie 1.5 mm from it, beyond the reach
+    assert got == pytest.approx(100.0 / 3.0)
