@@ -271,6 +271,13 @@ int m1_step(m1_ctx *ctx, const m1_numerics *num, int32_t kind,
 /* Kernel timing with HIP events on the context's stream (no reference
  * counterpart; the reference times the whole march, solver.py:102-114). */
 int m1_set_profiling(m1_ctx *ctx, int32_t enable);
+/* Form of the FP64 / FP64_FAST stage kernels (no reference counterpart):
+ * split = 1 (the default) runs each stage as a stencil-input pass, one face
+ * flux pass per axis and an update pass, each at several waves per SIMD;
+ * split = 0 runs the fused tile kernel (all of a stage in one launch, one
+ * wave per SIMD).  Both give the same bits; the split form holds 16 more
+ * doubles per cell of device memory. */
+int m1_set_stage_form(m1_ctx *ctx, int32_t split);
 int32_t m1_kernel_count(void);
 const char *m1_kernel_name(int32_t kernel);
 int m1_kernel_stats(m1_ctx *ctx, int32_t kernel, int64_t *launches,
@@ -347,6 +354,15 @@ int m1_batch_host_ms(m1_batch *batch, double *setup_ms);
  * columns; enable = 1 evaluates every cell of every tile of the boxes
  * instead.  Both give the same maximum bit for bit. */
 int m1_batch_set_cfl_all_tiles(m1_batch *batch, int32_t enable);
+/* Plain batch CFL with a tail floor (enable 1; no reference counterpart):
+ * a cell whose phi is at or below tile_threshold (normalised, the level of
+ * the tile set) counts with the speeds of the zero state, which is what it
+ * holds when its tile is left out.  The energy steps then depend on the
+ * cells above the threshold, the inflow and the materials present, not on
+ * which sub-threshold tiles are evaluated; sub-threshold cells may step up
+ * to ~1.2x past their own CFL bound (the largest M1 speed sum over the zero
+ * state's). */
+int m1_batch_set_cfl_tail_floor(m1_batch *batch, int32_t enable);
 
 /* Stage kernels on a persistent grid (the workgroups resident on the
  * device) that take tiles from eight per-XCD queues, stealing across XCDs
@@ -439,6 +455,12 @@ int m1_mlp_train(m1_mlp *model, const float *x, const float *y, int32_t n,
 int m1_batch_set_skip_model(m1_batch *batch, const m1_mlp *model, float decision,
                             int32_t horizon, int32_t infer_every, int32_t judge_every,
                             float judge_tol);
+/* Candidates of the network: guard_cells = 4 (the default) admits the tiles
+ * the reach rule adds while no cell within 4 cells of them holds phi above
+ * the threshold; 0 admits every tile the reach rule adds that holds no such
+ * cell itself (the network alone then decides on the tiles next to the
+ * beam).  Applies to the marches and to the recording below. */
+int m1_batch_set_skip_guard(m1_batch *batch, int32_t guard_cells);
 /* Record (features, label) samples of the dilation-only tiles during the
  * following marches, every `horizon` steps, up to `capacity` samples.
  * full_domain: evaluate every tile of the active boxes (labels from the
@@ -446,7 +468,9 @@ int m1_batch_set_skip_model(m1_batch *batch, const m1_mlp *model, float decision
 int m1_batch_record(m1_batch *batch, int32_t enable, int32_t full_domain,
                     int32_t horizon, int64_t capacity);
 /* *count = samples held; copies min(count, capacity) of them into x
- * (count, 11) and y. */
+ * (count, 11) and y: the label (1: the tile stayed at or below the threshold
+ * over the horizon), plus 2 when a cell within 4 cells of the tile was above
+ * the threshold when the sample was taken (possible with guard 0 only). */
 int m1_batch_samples(m1_batch *batch, float *x, float *y, int64_t capacity,
                      int64_t *count);
 /* Judge checks made and beamlets rejected since the model was set;

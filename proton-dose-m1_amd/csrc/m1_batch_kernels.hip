@@ -227,8 +227,12 @@ __device__ __forceinline__ void cfl_live_tile(const BatchCflArgs& a, const BoxDe
                 const long i = (long(x) * d.ny + y) * d.nz + z;
                 const bool zmin = z == 0;
                 const double pin = zmin ? spatial[i / d.nz] * pin_scale : 0.0;
+                // with the tail floor, a cell at or below the tile threshold
+                // counts as the zero state it becomes when its tile is left out
                 const double v =
-                    cfl_cell(U, i, N, c.S0[ids[i]], g, a.wave_mode, eps, zmin, pin, rf);
+                    a.tail_floor && !(U[i] > a.act_threshold)
+                        ? cfl_vacuum(c.S0[ids[i]], g, a.wave_mode, eps, zmin, pin, rf)
+                        : cfl_cell(U, i, N, c.S0[ids[i]], g, a.wave_mode, eps, zmin, pin, rf);
                 if (!(v == v))
                     bad = true;
                 else
@@ -415,9 +419,10 @@ __global__ void mask_kernel(BatchMaskArgs a)
             }
         }
     }
-    // Candidates of the network: added by the reach rule alone, with nothing
-    // above the threshold within TILE_GUARD cells.
+    // Candidates of the network: added by the reach rule alone and, with the
+    // guard, with nothing above the threshold within TILE_GUARD cells.
     bool cand = false;
+    uint8_t ring = 0;
     if (on && !forced && !self_on && (a.ring || a.pred)) {
         bool near = false;
         for (int dx = -1; dx <= 1 && !near; ++dx) {
@@ -438,10 +443,11 @@ __global__ void mask_kernel(BatchMaskArgs a)
                 }
             }
         }
-        cand = !near;
+        cand = a.guard ? !near : true;
+        ring = cand ? (near ? 2 : 1) : 0;
     }
     if (a.ring)
-        a.ring[t] = cand ? 1 : 0;
+        a.ring[t] = ring;
     if (a.pred && a.pred[t] > 0) {
         if (cand && st.model_on)
             on = false;
@@ -522,7 +528,9 @@ __global__ void label_kernel(BatchLabelArgs a)
         return;
     for (int i = 0; i < mlp::NIN; ++i)
         a.out_x[long(k) * mlp::NIN + i] = a.features[long(t) * mlp::NIN + i];
-    a.out_y[k] = a.wmax[t] <= a.threshold ? 1.f : 0.f;
+    // label (1: stayed at or below the threshold), plus 2 when a cell within
+    // TILE_GUARD cells was above it when the sample was taken
+    a.out_y[k] = (a.wmax[t] <= a.threshold ? 1.f : 0.f) + (a.cand[t] == 2 ? 2.f : 0.f);
 }
 
 // Judge: largest |full - skip| and |full| dose increment of each box.

@@ -212,6 +212,8 @@ struct m1_batch {
     const m1_mlp* model = nullptr;
     float decision = 0.5f, judge_tol = 1e-3f;
     int horizon = 8, judge_every = 64, infer_every = 4;
+    int guard_cells = 4;  // TILE_GUARD (4) or 0 (m1_batch_set_skip_guard)
+    bool cfl_tail_floor = false;  // m1_batch_set_cfl_tail_floor
     std::vector<uint8_t> rejected;
     std::vector<int32_t> rollback_to;  // step a rejected beamlet restarted from (-1: none)
     // two checkpoints per box (state and dose before a judged step)
@@ -710,7 +712,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             // Boxes with nothing to march leave the list at once.
             BatchMaskArgs ma{c->d_box,      c->d_step, c->d_tile_box, int(T), c->d_tile_max,
                              c->d_tile_face, c->d_flags[cur], c->d_flags[1 - cur], 0.f, 0, 0, 0, 1,
-                             nullptr,       nullptr,   nullptr,   nullptr};
+                             nullptr,       nullptr,   nullptr,   nullptr,   1};
             hip_check(c->K->mask(ma, s), "tile mask kernel");
             cur = 1 - cur;
             hip_check(c->K->compact(c->d_flags[cur], int(T), c->d_list, c->d_count, s),
@@ -837,7 +839,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                  use_model ? c->d_pred : nullptr,
                                  c->recording ? c->d_ring : nullptr,
                                  c->recording ? c->d_wmax : nullptr,
-                                 nullptr};
+                                 nullptr,
+                                 c->guard_cells > 0 ? 1 : 0};
                 Timed t(c->prof, s, K_TILE_MASK);
                 hip_check(c->K->mask(ma, s), "tile mask kernel (batch)");
                 hip_check(c->K->compact(c->d_flags[1 - k], int(T), c->d_list, c->d_count,
@@ -875,7 +878,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                 c->d_flags[k], c->d_tile_mats, c->d_cfl, c->d_red, n.wave,
                                 vacuum_aware ? 1 : 0, thr,
                                 by_list ? c->d_list : nullptr, c->d_count, c->d_boxmats,
-                                int(B), c->max_cols};
+                                int(B), c->max_cols, c->cfl_tail_floor ? 1 : 0};
                 Timed t(c->prof, s, K_CFL);
                 hip_check(c->K->cfl(ca, s), "CFL kernel (batch)");
             }
@@ -1098,7 +1101,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 slices(SLICE_RESET, d_judge);
                 BatchMaskArgs fa{c->d_box,   c->d_step, c->d_tile_box, int(T), c->d_tile_max,
                                  c->d_tile_face, c->d_flags[cur], c->d_flags_full, 0.f, 0, 0, 0, 1,
-                                 nullptr,      nullptr,   nullptr,       d_judge};
+                                 nullptr,      nullptr,   nullptr,       d_judge, 1};
                 hip_check(c->K->mask(fa, s), "judge tile set");
                 hip_check(c->K->compact(c->d_flags_full, int(T), c->d_list_full,
                                                 c->d_count_full, s),
@@ -1355,6 +1358,24 @@ int m1_batch_set_skip_model(m1_batch* c, const m1_mlp* model, float decision, in
         c->judge_every = judge_every;
         c->judge_tol = judge_tol;
         c->judged = c->rejections = 0;
+    });
+}
+
+int m1_batch_set_cfl_tail_floor(m1_batch* c, int32_t enable)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->cfl_tail_floor = enable != 0;
+    });
+}
+
+int m1_batch_set_skip_guard(m1_batch* c, int32_t guard_cells)
+{
+    return guarded(c, [&] {
+        if (guard_cells != 0 && guard_cells != TILE_GUARD)
+            throw ValueError{"guard_cells must be 0 or " + std::to_string(TILE_GUARD)};
+        sync(c);
+        c->guard_cells = guard_cells;
     });
 }
 

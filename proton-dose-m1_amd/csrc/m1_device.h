@@ -85,6 +85,14 @@ struct StageArgs {
     const double* bnd;      // (4, nx, ny) z-min boundary state of this stage
     const StepCoef* coef;   // device copy of the step scalars
     int wave_mode;          // 0 exact M1 speeds, 1 unit speeds
+    // Split form (all four set): the stencil input of stages 1 and 2 formed
+    // once per cell into win (4 x N) and the face fluxes of each axis in
+    // fx (4 x (nx+1) ny nz), fy (4 x nx (ny+1) nz), fz (4 x nx ny (nz+1));
+    // otherwise the fused tile kernel.
+    double* win;
+    double* fx;
+    double* fy;
+    double* fz;
 };
 
 struct CflArgs {
@@ -317,6 +325,8 @@ struct BatchCflArgs {
     unsigned long long* box_mats;
     int n_boxes;
     int max_columns;           // largest nx * ny of the boxes
+    int tail_floor;            // cells with normalised phi <= act_threshold take the
+                               // zero state's speeds (plain CFL)
 };
 
 struct BatchBoundaryArgs {
@@ -344,9 +354,13 @@ struct BatchMaskArgs {
     int all_on;               // no skipping: every tile of an active box
     uint8_t* pred;            // in/out (nullable): steps a tile stays left out
                               // (set by the network, counted down here)
-    uint8_t* ring;            // out (nullable): tiles the network may leave out
+    uint8_t* ring;            // out (nullable): tiles the network may leave out, 1 when
+                              // no cell within TILE_GUARD cells is above the
+                              // threshold, 2 when one is (guard 0 only)
     float* wmax;              // in/out (nullable): running max of tile_max
     const uint8_t* only;      // (nullable) boxes not flagged here get no tile
+    int guard;                // 1: candidates keep TILE_GUARD cells from any cell above
+                              // the threshold; 0: every tile the reach rule adds
 };
 
 // Tile features of the tile-skip network (11 per tile):

@@ -347,6 +347,92 @@ __device__ __forceinline__ double zmin_ghost(int k, double uin, double u0, doubl
            (35.0 / 4.0) * u3;
 }
 
+// Stage update of one cell from its flux divergence rhs and its own stencil
+// input w (energy.py:968-1128 stage algebra, dose quadrature, projection).
+template <int KIND, bool IFAC>
+__device__ __forceinline__ void stage_epilogue(const StageArgs& a, const StepCoef& sc, long i,
+                                               long N, const double rhs[4], const double w[4])
+{
+    const int m = a.ids[i];
+    const double dE = sc.dE;
+    if constexpr (KIND == STAGE_RHS) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            a.out[i + c * N] = rhs[c];
+    } else if constexpr (KIND == STAGE0) {
+        const double S0 = sc.S0[m];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            a.out[i + c * N] = rhs[c] * dE + S0 * w[c];
+    } else if constexpr (KIND == STAGE_FE) {
+        // energy.py:663-722
+        const double S0 = sc.S0[m], S1 = sc.S1[m];
+        double u[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            u[c] = (rhs[c] * dE + S0 * w[c]) / S1;
+        project(u[0], u[1], u[2], u[3]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            a.out[i + c * N] = u[c];
+        const double e = dE * w[0] * S0;
+        if (a.edep)
+            a.edep[i] = e;
+        if (a.dose)
+            a.dose[i] += e * MEV_TO_JOULE / sc.rho[m];
+    } else if constexpr (KIND == STAGE1) {
+        const double S0 = sc.S0[m];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double r = rhs[c];
+            if constexpr (IFAC)
+                r = r * (c == 0 ? sc.i1p[m] : sc.i1j[m]);
+            r = r * dE;
+            r = r + a.V1[i + c * N];
+            r = r * 0.25;
+            a.out[i + c * N] = 0.75 * (S0 * a.U[i + c * N]) + r;
+        }
+    } else {  // STAGE2
+        const double S0 = sc.S0[m], S1 = sc.S1[m], S2 = sc.S2[m];
+        double un[4], u[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            un[c] = a.U[i + c * N];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double r = rhs[c];
+            if constexpr (IFAC)
+                r = r * (c == 0 ? sc.i2p[m] : sc.i2j[m]);
+            r = r * dE;
+            r = r + a.src[i + c * N];
+            r = r * (2.0 / 3.0);
+            double v = (1.0 / 3.0) * (S0 * un[c]) + r;
+            if constexpr (IFAC)
+                v = v * (c == 0 ? sc.f1p[m] : sc.f1j[m]);
+            u[c] = v / S1;
+        }
+        project(u[0], u[1], u[2], u[3]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            a.U[i + c * N] = u[c];
+        // Third-order dose quadrature (energy.py:1121-1128, 1617-1621).
+        const double q0 = un[0] * S0;
+        double u1 = a.V1[i];
+        if constexpr (IFAC)
+            u1 = u1 * sc.f1p[m];
+        u1 = u1 / S1;
+        if (u1 < 0.0)
+            u1 = 0.0;
+        const double q1 = u1 * S1;
+        const double q2 = w[0] * S2;
+        const double e = dE * ((1.0 / 6.0) * q0 + (1.0 / 6.0) * q1 + (2.0 / 3.0) * q2);
+        if (a.edep)
+            a.edep[i] = e;
+        if (a.dose)
+            a.dose[i] += e * MEV_TO_JOULE / sc.rho[m];
+    }
+}
+
 // ------------------------------------------------------------ stage kernel
 template <int KIND, bool IFAC>
 __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a)
@@ -509,8 +595,6 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a)
 
         if (valid) {
             const long i = cidx(x, y, z);
-            const int m = a.ids[i];
-            const double dE = sc.dE;
             double rhs[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -525,82 +609,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a)
 #endif
                 rhs[c] = -(ax + ay + az);
             }
-            if constexpr (KIND == STAGE_RHS) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = rhs[c];
-            } else if constexpr (KIND == STAGE0) {
-                const double S0 = sc.S0[m];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = rhs[c] * dE + S0 * W[2][c];
-            } else if constexpr (KIND == STAGE_FE) {
-                // energy.py:663-722
-                const double S0 = sc.S0[m], S1 = sc.S1[m];
-                double u[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    u[c] = (rhs[c] * dE + S0 * W[2][c]) / S1;
-                project(u[0], u[1], u[2], u[3]);
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.out[i + c * N] = u[c];
-                const double e = dE * W[2][0] * S0;
-                if (a.edep)
-                    a.edep[i] = e;
-                if (a.dose)
-                    a.dose[i] += e * MEV_TO_JOULE / sc.rho[m];
-            } else if constexpr (KIND == STAGE1) {
-                const double S0 = sc.S0[m];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    double r = rhs[c];
-                    if constexpr (IFAC)
-                        r = r * (c == 0 ? sc.i1p[m] : sc.i1j[m]);
-                    r = r * dE;
-                    r = r + a.V1[i + c * N];
-                    r = r * 0.25;
-                    a.out[i + c * N] = 0.75 * (S0 * a.U[i + c * N]) + r;
-                }
-            } else {  // STAGE2
-                const double S0 = sc.S0[m], S1 = sc.S1[m], S2 = sc.S2[m];
-                double un[4], u[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    un[c] = a.U[i + c * N];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    double r = rhs[c];
-                    if constexpr (IFAC)
-                        r = r * (c == 0 ? sc.i2p[m] : sc.i2j[m]);
-                    r = r * dE;
-                    r = r + a.src[i + c * N];
-                    r = r * (2.0 / 3.0);
-                    double v = (1.0 / 3.0) * (S0 * un[c]) + r;
-                    if constexpr (IFAC)
-                        v = v * (c == 0 ? sc.f1p[m] : sc.f1j[m]);
-                    u[c] = v / S1;
-                }
-                project(u[0], u[1], u[2], u[3]);
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    a.U[i + c * N] = u[c];
-                // Third-order dose quadrature (energy.py:1121-1128, 1617-1621).
-                const double q0 = un[0] * S0;
-                double u1 = a.V1[i];
-                if constexpr (IFAC)
-                    u1 = u1 * sc.f1p[m];
-                u1 = u1 / S1;
-                if (u1 < 0.0)
-                    u1 = 0.0;
-                const double q1 = u1 * S1;
-                const double q2 = W[2][0] * S2;
-                const double e = dE * ((1.0 / 6.0) * q0 + (1.0 / 6.0) * q1 + (2.0 / 3.0) * q2);
-                if (a.edep)
-                    a.edep[i] = e;
-                if (a.dose)
-                    a.dose[i] += e * MEV_TO_JOULE / sc.rho[m];
-            }
+            stage_epilogue<KIND, IFAC>(a, sc, i, N, rhs, W[2]);
         }
 
 #pragma unroll
@@ -612,6 +621,162 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a)
             for (int c = 0; c < 4; ++c)
                 W[k][c] = W[k + 1][c];
     }
+}
+
+// ------------------------------------------------------ split stage kernels
+// The same stage as stage_kernel in three passes with one thread per cell or
+// per face and no tile state held across a march: (1) stages 1 and 2 form
+// their stencil input proj(C_k V_k / S_k) once per cell into `win`; (2) one
+// kernel per axis computes every face flux of that axis from the six cells
+// around the face (ghosts as in stage_kernel: zero gradient, quartic z-min
+// inflow) into fx / fy / fz; (3) the update kernel takes the divergence and
+// runs the stage epilogue.  Each pass needs a fraction of stage_kernel's
+// registers, so the double-precision WENO-Z / closure / HLL arithmetic runs
+// at several waves per SIMD instead of one; the flux arrays cost HBM traffic
+// that this VALU-bound arithmetic hides.  The arithmetic of every face and
+// cell is stage_kernel's, so both forms give the same bits.
+constexpr int SPLIT_THREADS = 256;
+#ifndef M1_F64_SPLIT_WAVES
+#define M1_F64_SPLIT_WAVES 2
+#endif
+
+template <int KIND, bool IFAC>
+__global__ void __launch_bounds__(SPLIT_THREADS) input_kernel(StageArgs a)
+{
+    const long N = long(a.g.nx) * a.g.ny * a.g.nz;
+    const long i = long(blockIdx.x) * SPLIT_THREADS + threadIdx.x;
+    if (i >= N)
+        return;
+    double u[4];
+    load_state<KIND, IFAC>(a, *a.coef, i, N, u);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        a.win[i + c * N] = u[c];
+}
+
+// Faces of AXIS: face f (0..n) lies between cells f-1 and f of that axis.
+template <int AXIS>
+__global__ void __launch_bounds__(SPLIT_THREADS, M1_F64_SPLIT_WAVES)
+face_kernel(StageArgs a, const double* __restrict__ in, double* __restrict__ flux)
+{
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const long N = long(nx) * ny * nz;
+    const int fny = ny + (AXIS == 1), fnz = nz + (AXIS == 2);
+    const long nf = long(nx + (AXIS == 0)) * fny * fnz;
+    const long t = long(blockIdx.x) * SPLIT_THREADS + threadIdx.x;
+    if (t >= nf)
+        return;
+    const int z = int(t % fnz);
+    const long r = t / fnz;
+    const int y = int(r % fny);
+    const int x = int(r / fny);
+    auto cidx = [&](int xx, int yy, int zz) -> long { return (long(xx) * ny + yy) * nz + zz; };
+    double s[4][6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if constexpr (AXIS == 0) {
+            const int xx = min(max(x - 3 + k, 0), nx - 1);
+            const long i = cidx(xx, y, z);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                s[c][k] = in[i + c * N];
+        } else if constexpr (AXIS == 1) {
+            const int yy = min(max(y - 3 + k, 0), ny - 1);
+            const long i = cidx(x, yy, z);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                s[c][k] = in[i + c * N];
+        } else {
+            const int zz = z - 3 + k;
+            if (zz < 0) {
+                // z-min inflow ghost from the boundary state and cells 0..3.
+                const long b = long(x) * ny + y;
+                const long m2 = long(nx) * ny;
+                const long i0 = cidx(x, y, 0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    s[c][k] = zmin_ghost(-zz, a.bnd[c * m2 + b], in[i0 + c * N],
+                                         in[i0 + 1 + c * N], in[i0 + 2 + c * N],
+                                         in[i0 + 3 + c * N]);
+            } else {
+                const long i = cidx(x, y, min(zz, nz - 1));
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    s[c][k] = in[i + c * N];
+            }
+        }
+    }
+    double F[4];
+    face_flux<AXIS>(s, a.wave_mode, F);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        flux[t + c * nf] = F[c];
+}
+
+template <int KIND, bool IFAC>
+__global__ void __launch_bounds__(SPLIT_THREADS) update_kernel(StageArgs a,
+                                                               const double* __restrict__ in)
+{
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const long N = long(nx) * ny * nz;
+    const long i = long(blockIdx.x) * SPLIT_THREADS + threadIdx.x;
+    if (i >= N)
+        return;
+    const int z = int(i % nz);
+    const int y = int((i / nz) % ny);
+    const int x = int(i / (long(ny) * nz));
+    const long nfx = long(nx + 1) * ny * nz, nfy = long(nx) * (ny + 1) * nz,
+               nfz = long(nx) * ny * (nz + 1);
+    const long ix = (long(x) * ny + y) * nz + z;            // face x (lower) of the cell
+    const long iy = (long(x) * (ny + 1) + y) * nz + z;      // face y (lower)
+    const long iz = (long(x) * ny + y) * (nz + 1) + z;      // face z (lower)
+    const long sx = long(ny) * nz;
+#if !M1_EXACT
+    const double idx = 1.0 / a.g.dx, idy = 1.0 / a.g.dy, idz = 1.0 / a.g.dz;
+#endif
+    double rhs[4], w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double fxl = a.fx[ix + c * nfx], fxh = a.fx[ix + sx + c * nfx];
+        const double fyl = a.fy[iy + c * nfy], fyh = a.fy[iy + nz + c * nfy];
+        const double fzl = a.fz[iz + c * nfz], fzh = a.fz[iz + 1 + c * nfz];
+#if M1_EXACT
+        const double ax = (fxh - fxl) / a.g.dx;
+        const double ay = (fyh - fyl) / a.g.dy;
+        const double az = (fzh - fzl) / a.g.dz;
+#else
+        const double ax = (fxh - fxl) * idx;
+        const double ay = (fyh - fyl) * idy;
+        const double az = (fzh - fzl) * idz;
+#endif
+        rhs[c] = -(ax + ay + az);
+        w[c] = in[i + c * N];
+    }
+    stage_epilogue<KIND, IFAC>(a, *a.coef, i, N, rhs, w);
+}
+
+template <int KIND, bool IFAC>
+hipError_t launch_split(const StageArgs& a, hipStream_t s)
+{
+    const long N = long(a.g.nx) * a.g.ny * a.g.nz;
+    const double* in = a.src;
+    if constexpr (KIND == STAGE1 || KIND == STAGE2) {
+        hipLaunchKernelGGL((input_kernel<KIND, IFAC>), dim3(unsigned((N + 255) / 256)),
+                           dim3(SPLIT_THREADS), 0, s, a);
+        in = a.win;
+    }
+    const long nfx = long(a.g.nx + 1) * a.g.ny * a.g.nz;
+    const long nfy = long(a.g.nx) * (a.g.ny + 1) * a.g.nz;
+    const long nfz = long(a.g.nx) * a.g.ny * (a.g.nz + 1);
+    hipLaunchKernelGGL(face_kernel<0>, dim3(unsigned((nfx + 255) / 256)), dim3(SPLIT_THREADS), 0,
+                       s, a, in, a.fx);
+    hipLaunchKernelGGL(face_kernel<1>, dim3(unsigned((nfy + 255) / 256)), dim3(SPLIT_THREADS), 0,
+                       s, a, in, a.fy);
+    hipLaunchKernelGGL(face_kernel<2>, dim3(unsigned((nfz + 255) / 256)), dim3(SPLIT_THREADS), 0,
+                       s, a, in, a.fz);
+    hipLaunchKernelGGL((update_kernel<KIND, IFAC>), dim3(unsigned((N + 255) / 256)),
+                       dim3(SPLIT_THREADS), 0, s, a, in);
+    return hipGetLastError();
 }
 
 // -------------------------------------------------------------- CFL kernel
@@ -797,6 +962,22 @@ __global__ void scale_kernel(ScaleArgs a)
 
 hipError_t launch_stage(int kind, bool ifac, const StageArgs& a, hipStream_t s)
 {
+    if (a.fx && a.fy && a.fz && a.win) {
+        switch (kind) {
+        case STAGE0:
+            return launch_split<STAGE0, false>(a, s);
+        case STAGE1:
+            return ifac ? launch_split<STAGE1, true>(a, s) : launch_split<STAGE1, false>(a, s);
+        case STAGE2:
+            return ifac ? launch_split<STAGE2, true>(a, s) : launch_split<STAGE2, false>(a, s);
+        case STAGE_RHS:
+            return launch_split<STAGE_RHS, false>(a, s);
+        case STAGE_FE:
+            return launch_split<STAGE_FE, false>(a, s);
+        default:
+            return hipErrorInvalidValue;
+        }
+    }
     const dim3 grid((a.g.nz + TZ - 1) / TZ, (a.g.ny + TY - 1) / TY, (a.g.nx + TX - 1) / TX);
     const dim3 block(NT);
 #define M1_LAUNCH(K, I) hipLaunchKernelGGL((stage_kernel<K, I>), grid, block, 0, s, a)

@@ -61,6 +61,10 @@ struct m1_ctx {
     double *U = nullptr, *V1 = nullptr, *V2 = nullptr, *work = nullptr, *dose = nullptr,
            *edep = nullptr, *bnd = nullptr, *spatial = nullptr;
     uint8_t* ids = nullptr;
+    // Split form of the FP64 stages (stencil input and face fluxes, sized for
+    // the whole grid), allocated on first use; split_stages selects it.
+    double *win = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    bool split_stages = true;
     std::vector<double> h_spatial;  // inflow profile (nx, ny), host copy
     // The cells the FP64 kernels march: the whole grid, or the current box
     // of the moving active domain (its ids and inflow profile cut out).
@@ -112,10 +116,12 @@ void m1_ctx::free_problem()
                     (void*)V2_32, (void*)work32, (void*)dose_hi, (void*)dose_lo, (void*)edep32,
                     (void*)bnd32, (void*)d_compute[0], (void*)d_compute[1], (void*)d_tile_max,
                     (void*)d_tile_face, (void*)d_nactive, (void*)box_ids, (void*)box_act,
-                    (void*)box_spatial, (void*)box_dose})
+                    (void*)box_spatial, (void*)box_dose, (void*)win, (void*)fx,
+                    (void*)fy, (void*)fz})
         if (p)
             (void)hipFree(p);
     U = V1 = V2 = work = dose = edep = bnd = spatial = nullptr;
+    win = fx = fy = fz = nullptr;
     U32 = V1_32 = V2_32 = work32 = dose_hi = dose_lo = edep32 = bnd32 = nullptr;
     d_compute[0] = d_compute[1] = nullptr;
     d_tile_max = d_tile_face = nullptr;
@@ -185,13 +191,33 @@ hipError_t stage(const m1_ctx* c, const Num& n, int kind, bool ifac, const m1::S
     return m1::fast::launch_stage(kind, ifac, a, c->stream);
 }
 
-m1::StageArgs base_args(const m1_ctx* c, const Num& n)
+// Buffers of the split stage form (m1_kernels.hip launch_split), sized for
+// the whole grid so that every box of the moving domain fits.
+void ensure_split(m1_ctx* c)
+{
+    if (c->win)
+        return;
+    const size_t nx = size_t(c->nx), ny = size_t(c->ny), nz = size_t(c->nz);
+    dev_alloc(&c->win, 4 * nx * ny * nz, "hipMalloc stencil input");
+    dev_alloc(&c->fx, 4 * (nx + 1) * ny * nz, "hipMalloc x fluxes");
+    dev_alloc(&c->fy, 4 * nx * (ny + 1) * nz, "hipMalloc y fluxes");
+    dev_alloc(&c->fz, 4 * nx * ny * (nz + 1), "hipMalloc z fluxes");
+}
+
+m1::StageArgs base_args(m1_ctx* c, const Num& n)
 {
     m1::StageArgs a{};
     a.g = dims(c);
     a.ids = c->dom.ids;
     a.coef = c->d_step;
     a.wave_mode = n.wave;
+    if (c->split_stages) {
+        ensure_split(c);
+        a.win = c->win;
+        a.fx = c->fx;
+        a.fy = c->fy;
+        a.fz = c->fz;
+    }
     return a;
 }
 
@@ -1266,6 +1292,14 @@ int m1_set_profiling(m1_ctx* ctx, int32_t enable)
     return guarded(ctx, [&] {
         sync(ctx);
         ctx->prof.on = enable != 0;
+    });
+}
+
+int m1_set_stage_form(m1_ctx* ctx, int32_t split)
+{
+    return guarded(ctx, [&] {
+        sync(ctx);
+        ctx->split_stages = split != 0;
     });
 }
 

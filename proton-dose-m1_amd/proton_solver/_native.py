@@ -97,6 +97,7 @@ SIGNATURES = [
     ("m1_step", C.c_int, [C.c_void_p, C.POINTER(m1_numerics), C.c_int32, C.c_double,
                           C.c_double, _dp]),
     ("m1_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_set_stage_form", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_kernel_count", C.c_int32, []),
     ("m1_kernel_name", C.c_char_p, [C.c_int32]),
     ("m1_tile_shape", None, [C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -132,6 +133,8 @@ SIGNATURES = [
                                       C.POINTER(C.c_char_p)]),
     ("m1_batch_set_skip_model", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
                                           C.c_int32, C.c_int32, C.c_float]),
+    ("m1_batch_set_cfl_tail_floor", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_set_skip_guard", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
     ("m1_batch_samples", C.c_int, [C.c_void_p, _fp, _fp, C.c_int64, C.POINTER(C.c_int64)]),
     ("m1_batch_skip_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -435,6 +438,11 @@ class Context:
         return edep
 
     # --- profiling -------------------------------------------------------
+    def set_stage_form(self, split: bool):
+        """FP64 stages as split passes (True, the default) or the fused tile
+        kernel (False); the results are the same bits."""
+        self._check(self.lib.m1_set_stage_form(self.h, int(bool(split))))
+
     def set_profiling(self, on: bool):
         self._check(self.lib.m1_set_profiling(self.h, int(bool(on))))
 
@@ -624,6 +632,15 @@ class Batch:
                                                      int(infer_every), int(judge_every),
                                                      float(judge_tol)))
         self.model = model  # the library keeps a pointer: keep the model alive
+
+    def set_cfl_tail_floor(self, enable: bool):
+        """Sub-threshold cells count with the zero state's speeds in the CFL."""
+        self._check(self.lib.m1_batch_set_cfl_tail_floor(self.h, int(bool(enable))))
+
+    def set_skip_guard(self, guard_cells: int):
+        """Network candidates: 4 keeps them 4 cells from any cell above the
+        threshold (the default), 0 admits every tile the reach rule adds."""
+        self._check(self.lib.m1_batch_set_skip_guard(self.h, int(guard_cells)))
 
     def record(self, enable=True, full_domain=True, horizon=8, capacity=1 << 20):
         """Record (features, label) samples during the following marches."""

@@ -58,9 +58,11 @@ def sample_beamlets(beamlets, n):
 
 
 def autotune(phantom, beamlets, config, sample=12, shapes=None, modes=MODES, device=None,
-             log=None):
+             log=None, setup=None):
     """Fastest (tile shape, launch) on a sample of the batch; returns a Choice
-    whose table lists every timed candidate."""
+    whose table lists every timed candidate.  setup(handle), when given, is
+    applied to every candidate batch before it is timed (e.g. the tile-skip
+    network the timed march uses)."""
     from .batch import prepare_batch
 
     sub = sample_beamlets(beamlets, sample)
@@ -79,6 +81,8 @@ def autotune(phantom, beamlets, config, sample=12, shapes=None, modes=MODES, dev
     handles = {}
     for n_i, i in enumerate(idx):
         handle, _ = prepare_batch(phantom, sub, config, device=device, tile_shape=i)
+        if setup is not None:
+            setup(handle)
         if n_i == 0:
             timed(handle, "per_tile")  # first launches of the process
         sec = timed(handle, "per_tile")

@@ -359,3 +359,28 @@ def batch_march(phantom: Phantom, beamlets, config, boxes=None, want_dose=True,
     num = _native.numerics_struct(config)
     dose, steps = handle.march(num, want_dose)
     return BatchResult(dose=dose, steps=steps, boxes=list(handle.boxes), handle=handle)
+
+
+# ------------------------------------------------------- one beam, whole grid
+def interface_workload(offset_m=0.0, shape=(160, 160, 480), spacing=(0.0005,) * 3):
+    """The lateral water / cortical-bone interface workload: a 160 x 160 x 480
+    grid at 0.5 mm whose x > 0 half is bone (1.85 g/cm^3), and one 200 MeV
+    beam (sigma 3 mm, sigma_E 1 MeV) entering at x = -2 mm + offset_m, y = 0.
+    Returned as (phantom, [beamlet], [box]) with the box the whole grid, so
+    the batched march (and its tile-skip network) solves it as one beamlet
+    on the full domain."""
+    nx, ny, nz = shape
+    x = (np.arange(nx) + 0.5) * spacing[0] - 0.5 * nx * spacing[0]
+    ids = np.zeros(shape, dtype=np.int32)
+    ids[x >= 0.0] = 1  # half-open on cell centres, as a Box(0, ...) voxelises
+    phantom = Phantom(ids=ids, materials=(Material.WATER, Material.BONE_COMPACT),
+                      spacing=tuple(float(d) for d in spacing))
+    beamlet = Beamlet(energy=200.0, sigma=(0.003, 0.003), center=(-0.002 + offset_m, 0.0))
+    return phantom, [beamlet], [(0, 0, nx, ny, nz)]
+
+
+def interface_offsets(n, seed=3, spread_m=0.004):
+    """Seeded beam offsets along x, U(-spread, spread) (the training runs of
+    the interface workload's tile-skip network)."""
+    rng = np.random.default_rng(seed)
+    return [float(v) for v in rng.uniform(-spread_m, spread_m, n)]

@@ -1,0 +1,80 @@
+"""The 256-beamlet workload at its full size on an MI355X.
+
+256 pencil beamlets (E ~ U(70, 230) MeV, sigma ~ U(2, 6) mm, entry points
+~ U(-30, 30) mm, seeds 40 / 4) in one batch through the 200 x 200 x 300 water
+phantom at 1 mm with 12 bone (1.85) / lung (0.26 g/cm^3) ellipsoids, each
+beamlet on its 64 x 64 x depth box, Wentzel + nuclear, IF-SSPRK3, FP32 bulk
+with tile skipping, the tile-skip network and its judge: the configuration
+bench.py measures.  Eight of its beamlets -- six at evenly spaced energy ranks
+and the two whose maximum dose moved most under a box-only CFL bound (109
+and 127) -- are marched by the plain FP64 march on the whole phantom (bitwise
+the CPU oracle) and each batch dose must be within 0.5 % of its maximum dose
+with gamma(1 %/1 mm) >= 99.5 %, the acceptance of the mixed-precision march.
+The FP64 dose outside the gamma region (the box and a 4-cell margin) must be
+below 1e-3 of its maximum, so the region holds every dose that matters.
+"""
+
+import numpy as np
+import pytest
+
+from proton_solver import _native, metrics, tileskip
+from proton_solver.batch import batch_workload, batch_workload_config, full_problem, prepare_batch
+from test_gpu_parity import say
+
+pytestmark = pytest.mark.gpu
+
+STRESS = (109, 127)
+
+
+def _picks(beamlets):
+    order = np.argsort([b.energy for b in beamlets], kind="stable")
+    ranks = [int(order[int(round(q * (len(beamlets) - 1)))])
+             for q in np.linspace(0.0, 1.0, 6)]
+    return sorted(set(ranks) | set(STRESS), key=lambda b: beamlets[b].energy)
+
+
+@pytest.fixture(scope="module")
+def batch256():
+    phantom, beamlets = batch_workload(n_beamlets=256)
+    cfg = batch_workload_config()
+    # the tile shape the autotuner picks for this batch (8 x 16 x 16)
+    shape = [tuple(t) for t in _native.batch_tile_shapes()].index((8, 16, 16))
+    handle, boxes = prepare_batch(phantom, beamlets, cfg, tile_shape=shape)
+    use_net = tileskip.WEIGHTS_FILE.exists()
+    model = tileskip.load_model(0) if use_net else None
+    tileskip.enable(handle, model)
+    say(f"256-beamlet batch: marching (network {'on' if use_net else 'off'})")
+    _, steps = handle.march(_native.numerics_struct(cfg), want_dose=False)
+    say(f"256-beamlet batch: steps {int(steps.min())}..{int(steps.max())}")
+    picks = _picks(beamlets)
+    doses = {b: handle.beamlet_dose(b) for b in picks}
+    ctx = _native.Context(0)
+    return phantom, beamlets, boxes, cfg, steps, doses, ctx, model
+
+
+@pytest.mark.parametrize("rank", range(8))
+def test_batch256_beamlet_against_plain_fp64(batch256, rank):
+    phantom, beamlets, boxes, cfg, steps, doses, ctx, _ = batch256
+    picks = _picks(beamlets)
+    assert len(picks) == 8 and all(s in picks for s in STRESS)
+    b = picks[rank]
+    say(f"beamlet {b} ({beamlets[b].energy:.1f} MeV): plain FP64 march on the whole phantom")
+    ctx.set_problem(*full_problem(phantom, beamlets[b]))
+    ctx.set_state(None)
+    E64, _ = ctx.march(_native.numerics_struct(cfg, _native.FP64))
+    ref = ctx.get_dose()
+    x0, y0, nx, ny, nz = boxes[b]
+    got = np.zeros(phantom.shape)
+    got[x0:x0 + nx, y0:y0 + ny, :nz] = doses[b]
+    sl = (slice(max(x0 - 4, 0), x0 + nx + 4), slice(max(y0 - 4, 0), y0 + ny + 4),
+          slice(0, min(nz + 4, phantom.shape[2])))
+    outside = ref.copy()
+    outside[sl] = 0.0
+    err = metrics.maximum_dose_relative_difference(got, ref)
+    gamma = metrics.gamma_pass_rate(np.ascontiguousarray(got[sl]), np.ascontiguousarray(ref[sl]),
+                                    phantom.spacing, 1.0, 1.0, 10.0, device=0)
+    say(f"beamlet {b}: FP64 {len(E64) - 1} steps, batch {int(steps[b])}; max-dose error "
+        f"{err:+.2e}, gamma {gamma:.3f} %, outside {outside.max() / ref.max():.1e}")
+    assert abs(err) <= 5e-3
+    assert gamma >= 99.5
+    assert outside.max() <= 1e-3 * ref.max()

@@ -243,3 +243,52 @@ def test_other_schemes_march(scheme):
     (U, dose, E, _), (U_r, dose_r, E_r, _) = march_both(cfg, EXACT)
     assert np.array_equal(dose, dose_r)
     assert np.array_equal(U, U_r)
+
+
+@pytest.mark.parametrize("precision", [EXACT, FAST], ids=["fp64", "fp64_fast"])
+def test_split_and_fused_stage_forms_give_the_same_bits(precision):
+    """The split FP64 stage form (stencil-input, per-axis face and update
+    passes; the default) and the fused tile kernel: same spatial operator,
+    same single steps of every kind, same march, bit for bit in FP64 (and to
+    rounding in FP64_FAST)."""
+    bone = Box(0.0, 0.006, -0.006, 0.006, 0.004, 0.012, Material.BONE_COMPACT)
+    cfg = small_config(shape=(12, 14, 36), physics=Physics(scattering=Scattering.WENTZEL,
+                                                           nuclear=True), objects=[bone],
+                       energy=40.0)
+    grid, mm, inflow, ctx, _ = setup(cfg)
+    U = random_state((12, 14, 36), 11)
+    forms = {}
+    for split in (True, False):
+        ctx.set_stage_form(split)
+        num = _native.numerics_struct(cfg, precision)
+        ctx.set_state(U)
+        rhs = ctx.spatial_operator(num, 30.0)
+        steps = []
+        for kind in (0, 1, 2, 3):
+            ctx.set_state(U)
+            edep = ctx.step(num, kind, 30.0, 0.2)
+            steps.append((ctx.get_state(), edep))
+        ctx.set_state(None)
+        E, _ = ctx.march(num)
+        forms[split] = (rhs, steps, E, ctx.get_dose(), ctx.get_state())
+        say(f"stage form split={split}: {len(E) - 1} steps, max dose {forms[split][3].max():.6e}")
+    a, b = forms[True], forms[False]
+    if precision == EXACT:
+        assert np.array_equal(a[0], b[0])
+        for (ua, ea), (ub, eb) in zip(a[1], b[1]):
+            assert np.array_equal(ua, ub) and np.array_equal(ea, eb)
+        assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+        assert np.array_equal(a[4], b[4])
+        return
+    # FP64_FAST lets the compiler fuse multiply-adds, and it fuses the same
+    # face arithmetic differently when inlined into different kernels: the
+    # tolerance of the FP64_FAST operator tests above (near free streaming
+    # the eigenvalue discriminant cancels).
+    def close(x, y, rel):
+        return np.abs(x - y).max() <= rel * np.abs(y).max()
+
+    assert close(a[0], b[0], 1e-8)
+    for (ua, ea), (ub, eb) in zip(a[1], b[1]):
+        assert close(ua, ub, 1e-8) and close(ea, eb, 1e-8)
+    assert len(a[2]) == len(b[2]) and close(a[2], b[2], 1e-12)
+    assert close(a[3], b[3], 1e-8)
