@@ -55,8 +55,9 @@ constexpr int TZ = F32_TZ;
 constexpr int TX = F32_TX;
 static_assert(TY * TZ == 256, "one thread per y-z cell of a tile");
 static_assert(TX * TY <= 256, "the batch CFL kernel maps one thread to each z-min column");
-static_assert(TY + TZ <= 256, "one thread per upper tile face");
 constexpr int NT = TY * TZ;
+static_assert(TY <= 64 && TZ <= 64 && NT >= 128,
+              "upper tile faces: z in the first wave, y in the second");
 constexpr int PY = TY + 6;
 constexpr int PZ = TZ + 6;
 constexpr int NHALO = 6 * TZ + 6 * TY;
@@ -437,16 +438,6 @@ __device__ __forceinline__ void face_hll(const float Ls[4], const float Rs[4], i
         const FaceF cr = face_state(Rs[0], Rs[1], Rs[2], Rs[3]);
         hll_f<AXIS>(cl, cr, wave, float(eeps), F);
     }
-}
-
-template <bool D>
-__device__ __forceinline__ void face_hll_yz(bool zaxis, const float L[4], const float R[4],
-                                            int wave, double eeps, float F[4])
-{
-    if (zaxis)
-        face_hll<2, D>(L, R, wave, eeps, F);
-    else
-        face_hll<1, D>(L, R, wave, eeps, F);
 }
 
 // Stencil input of one cell: U directly, or proj(C_k V_k / S_k) for stages 1, 2.
@@ -835,13 +826,17 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
             // y lanes: column h_line, centre y = -1 or TY
 #pragma unroll 1
             for (int c = 0; c < 4; ++c) {
+                // the upper cell's lower state is the mirrored stencil's left
+                // state: reverse the operands, so that every lane runs one
+                // weno_left (the lanes of both kinds share a wave)
                 float v[5];
 #pragma unroll
-                for (int k = 0; k < 5; ++k)
-                    v[k] = hz_lane ? P[c][h_line + 3][(h_high ? TZ + 1 : 0) + k]
-                                   : P[c][(h_high ? TY + 1 : 0) + k][h_line + 3];
-                const float st = h_high ? weno_left(v[4], v[3], v[2], v[1], v[0], weps)
-                                        : weno_left(v[0], v[1], v[2], v[3], v[4], weps);
+                for (int k = 0; k < 5; ++k) {
+                    const int kk = h_high ? 4 - k : k;
+                    v[k] = hz_lane ? P[c][h_line + 3][(h_high ? TZ + 1 : 0) + kk]
+                                   : P[c][(h_high ? TY + 1 : 0) + kk][h_line + 3];
+                }
+                const float st = weno_left(v[0], v[1], v[2], v[3], v[4], weps);
                 if (hz_lane) {
                     if (h_high)
                         Hz[c][h_line] = st;
@@ -856,8 +851,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
             }
         }
         __syncthreads();
-        // Fluxes: each cell's lower faces; the first TY + TZ lanes also the
-        // upper faces of the tile.
+        // Fluxes: each cell's lower faces, then the upper faces of the tile.
         {
             float L[4], R[4], F[4];
 #pragma unroll
@@ -879,23 +873,31 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
             for (int c = 0; c < 4; ++c)
                 Fy[c][ty][tz] = F[c];
         }
-        if (tid < TY + TZ) {
-            const bool zaxis = tid < TY;
-            const int line = zaxis ? tid : tid - TY;
+        // Upper faces of the tile: the z faces in the first wave, the y
+        // faces in the second, so that no wave runs both flux variants.
+        if (tid < TY) {
             float L[4], R[4], F[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                L[c] = zaxis ? Fz[c][line][TZ] : Fy[c][TY][line];
-                R[c] = zaxis ? Hz[c][line] : Hy[c][line];
+                L[c] = Fz[c][tid][TZ];
+                R[c] = Hz[c][tid];
             }
-            face_hll_yz<D>(zaxis, L, R, wave, eeps, F);
+            face_hll<2, D>(L, R, wave, eeps, F);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                Fz[c][tid][TZ] = F[c];
+        } else if (tid >= 64 && tid < 64 + TZ) {
+            const int line = tid - 64;
+            float L[4], R[4], F[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                if (zaxis)
-                    Fz[c][line][TZ] = F[c];
-                else
-                    Fy[c][TY][line] = F[c];
+                L[c] = Fy[c][TY][line];
+                R[c] = Hy[c][line];
             }
+            face_hll<1, D>(L, R, wave, eeps, F);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                Fy[c][TY][line] = F[c];
         }
         __syncthreads();
 
