@@ -51,11 +51,14 @@ def batch_shapes():
 def _units():
     waves = [f"-DM1_STAGE_WAVES={int(os.environ['M1_STAGE_WAVES'])}"] \
         if os.environ.get("M1_STAGE_WAVES") else []
+    # waves per SIMD of the FP64 face kernels (default in m1_kernels.hip)
+    fwaves = [f"-DM1_F64_SPLIT_WAVES={int(os.environ['M1_F64_SPLIT_WAVES'])}"] \
+        if os.environ.get("M1_F64_SPLIT_WAVES") else []
     return [
         (CSRC / "m1_kernels.hip", BUILD / "m1_kernels_exact.o",
-         ["-DM1_EXACT=1", "-ffp-contract=off"]),
+         ["-DM1_EXACT=1", "-ffp-contract=off", *fwaves]),
         (CSRC / "m1_kernels.hip", BUILD / "m1_kernels_fast.o",
-         ["-DM1_EXACT=0", "-ffp-contract=fast"]),
+         ["-DM1_EXACT=0", "-ffp-contract=fast", *fwaves]),
         # float kernels: fused multiply-adds only where written (fmaf), so the
         # four moments of a cell round alike (m1_f32_common.h)
         (CSRC / "m1_kernels_f32.hip", BUILD / "m1_kernels_f32.o", ["-ffp-contract=off", *waves]),
