@@ -386,6 +386,30 @@ int m1_batch_set_graphs(m1_batch *batch, int32_t enable);
 /* Steps of the last march replayed from a graph; *capture_error (nullable)
  * is empty, or why a capture failed and the march fell back to launches. */
 int m1_batch_graph_info(m1_batch *batch, int64_t *graph_steps, const char **capture_error);
+/* Device-side energy stepping (enable 1; no reference counterpart; replaces
+ * the per-step host work of energy_march, numerics/energy.py:1556-1624, for
+ * the batch): each beamlet's CFL step (energy.py:640-660), its step scalars
+ * (stopping_power.py:141-228, scattering.py:267-306, nuclear.py:202-233,
+ * energy.py:273-310, boundary.py:82-114), its energy, the judge's decisions
+ * and the box flags are computed by kernels, and the host enqueues steps
+ * without waiting for them (it reads each step's status a few steps later).
+ * Sums, products, quotients and square roots round as the host's; the
+ * exponentials are the device library's, so results agree with host-side
+ * stepping to the last bits of the exponentials, not always bit for bit.
+ * With the tile-skip network, beamlets are judged on a common schedule
+ * (every judge_every-th step of the batch) rather than per beamlet, and not
+ * on their last step.  Recording marches keep host-side stepping. */
+int m1_batch_set_device_steps(m1_batch *batch, int32_t enable);
+/* Device evaluation of the step physics of the bound problem's material
+ * `material` at n energies (test access; no reference counterpart): kind 0
+ * S(E) (stopping_power.py:141-228), 1 the transport coefficient of
+ * `scattering` (scattering.py:65-153), 2 the removal coefficient
+ * (nuclear.py:94-138), 3 the scattering optical depth over [E - dE, E]
+ * (scattering.py:267-306), 4 the nuclear optical depth (nuclear.py:202-233);
+ * interpolation as m1_numerics.coefficient_interpolation. */
+int m1_batch_physics_probe(m1_batch *batch, int32_t material, int32_t kind, int32_t scattering,
+                           int32_t interpolation, const double *energies, double dE, int32_t n,
+                           double *out);
 
 /* ------------------------------------------------------------------
  * Tile-skip network (no reference counterpart): 11 inputs per workgroup

@@ -71,6 +71,7 @@ def _units():
           for x, y, z in batch_shapes()],
         (CSRC / "m1_batch_march.hip", BUILD / "m1_batch_march.o", ["-ffp-contract=off", *waves]),
         (CSRC / "m1_mlp.hip", BUILD / "m1_mlp.o", ["-ffp-contract=off"]),
+        (CSRC / "m1_batch_steps.hip", BUILD / "m1_batch_steps.o", ["-ffp-contract=off"]),
     ]
 
 

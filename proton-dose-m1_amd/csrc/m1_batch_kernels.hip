@@ -86,7 +86,8 @@ __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
     stage_tile<KIND, IFAC, D, true>(v);
 }
 
-// One workgroup per list entry, or (a.queue set) a persistent grid pulling
+// One workgroup per list entry (strided: per entry of an estimated length),
+// or (a.queue set) a persistent grid pulling
 // entries from the per-XCD queues: a workgroup of XCD x (workgroups are
 // dispatched round-robin over the XCDs) drains slice x first, keeping halo
 // re-reads of neighbouring tiles in that XCD's L2, then steals from the
@@ -96,7 +97,20 @@ template <int KIND, bool IFAC, bool D>
 __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArgs a)
 {
     if (!a.queue) {
-        run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(int(blockIdx.x), a.n_list)]);
+        if (!a.strided) {
+            run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(int(blockIdx.x), a.n_list)]);
+            return;
+        }
+        // Grid of an estimated list length (the length is read here): slot i
+        // as in the exact launch while the grid covers the list, and the
+        // workgroups take further slots grid-stride when it does not (the
+        // grid is a multiple of 8, so a slot stays on its XCD).
+        const int n = *a.count;
+        for (int i = int(blockIdx.x); i < n; i += int(gridDim.x)) {
+            run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(i, n)]);
+            if (i + int(gridDim.x) < n)
+                __syncthreads();  // the next tile reuses LDS
+        }
         return;
     }
     __shared__ int s_slot;

@@ -25,9 +25,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -36,6 +38,7 @@
 #include <thread>
 #include <vector>
 
+#include "m1_batch_steps.h"
 #include "m1_host.h"
 #include "m1_mlp.h"
 
@@ -233,6 +236,18 @@ struct m1_batch {
     float *d_rx = nullptr, *d_ry = nullptr;
     int* d_rcount = nullptr;
 
+    // device-side energy stepping (m1_batch_set_device_steps)
+    bool device_steps = false;
+    DevMaterial* d_mats = nullptr;  // material tables on the device
+    double* d_tables = nullptr;
+    BeamletConst* d_bc = nullptr;
+    BeamletState* d_bst = nullptr;
+    DevCounters* d_ctr = nullptr;
+    StepStatus* h_ring = nullptr;   // pinned: status of the last RING steps
+    StepStatus* d_status = nullptr; // its device address
+    static constexpr int RING = 4;
+    hipEvent_t ring_ev[RING] = {};
+
     Profiler prof;
     double host_ms = 0.0;
     bool marched = false;
@@ -290,6 +305,14 @@ void m1_batch::free_problem()
     for (void* p : dev)
         if (p)
             (void)hipFree(p);
+    for (void* p : {(void*)d_mats, (void*)d_tables, (void*)d_bc, (void*)d_bst, (void*)d_ctr})
+        if (p)
+            (void)hipFree(p);
+    d_mats = nullptr;
+    d_tables = nullptr;
+    d_bc = nullptr;
+    d_bst = nullptr;
+    d_ctr = nullptr;
     void* host[] = {h_coef, h_cfl, h_coef32, h_step, h_red, h_boxmats, h_count};
     for (void* p : host)
         if (p)
@@ -323,6 +346,11 @@ m1_batch::~m1_batch()
         (void)hipStreamSynchronize(stream);
     free_problem();
     prof.release();
+    if (h_ring)
+        (void)hipHostFree(h_ring);
+    for (hipEvent_t& e : ring_ev)
+        if (e)
+            (void)hipEventDestroy(e);
     if (d_gate)
         (void)hipFree(d_gate);
     if (h_gate)
@@ -369,6 +397,70 @@ int inflow_on(const m1_beamlet& b, const Num& n, double energy_next)
     return std::exp(-0.5 * d * d) > n.tile_threshold ? 1 : 0;
 }
 
+// Material tables of the problem on the device (one buffer of doubles, and
+// the DevMaterial records pointing into it).
+void upload_tables(m1_batch* c)
+{
+    std::vector<double> buf;
+    std::vector<DevMaterial> recs(c->mats.size());
+    auto put = [&](const Table& t, DevTable& d, std::vector<std::array<long, 3>>& offs) {
+        std::array<long, 3> o{-1, -1, -1};
+        o[0] = long(buf.size());
+        buf.insert(buf.end(), t.E.begin(), t.E.end());
+        o[1] = long(buf.size());
+        buf.insert(buf.end(), t.v.begin(), t.v.end());
+        if (!t.slope.empty()) {
+            o[2] = long(buf.size());
+            buf.insert(buf.end(), t.slope.begin(), t.slope.end());
+        }
+        d.n = int(t.E.size());
+        d.pad = 0;
+        offs.push_back(o);
+    };
+    std::vector<std::array<long, 3>> offs;
+    for (size_t m = 0; m < c->mats.size(); ++m) {
+        put(c->mats[m].sp, recs[m].sp, offs);
+        put(c->mats[m].tr, recs[m].tr, offs);
+        put(c->mats[m].nu, recs[m].nu, offs);
+        recs[m].density = c->mats[m].density;
+        recs[m].radiation_length = c->mats[m].radiation_length;
+    }
+    dev_alloc(&c->d_tables, buf.size(), "hipMalloc material tables");
+    hip_check(hipMemcpy(c->d_tables, buf.data(), buf.size() * sizeof(double),
+                        hipMemcpyHostToDevice),
+              "table upload");
+    size_t k = 0;
+    for (auto& r : recs)
+        for (DevTable* d : {&r.sp, &r.tr, &r.nu}) {
+            const auto& o = offs[k++];
+            d->E = c->d_tables + o[0];
+            d->v = c->d_tables + o[1];
+            d->slope = o[2] >= 0 ? c->d_tables + o[2] : nullptr;
+        }
+    dev_alloc(&c->d_mats, recs.size(), "hipMalloc material records");
+    hip_check(hipMemcpy(c->d_mats, recs.data(), recs.size() * sizeof(DevMaterial),
+                        hipMemcpyHostToDevice),
+              "table upload");
+}
+
+// The host's ValueError / NonFinite for the first error of a device march.
+[[noreturn]] void throw_device_error(const m1_batch* c, const DevError& e)
+{
+    if (e.code == DEVERR_NONFINITE)
+        throw NonFinite{"non-finite M1 state detected by the CFL reduction"};
+    if (e.code == DEVERR_SLOPES)
+        throw ValueError{"PCHIP interpolation needs nodal slopes"};
+    const char* what[3] = {"stopping-power table", "Wentzel table", "nuclear table"};
+    const int tab = e.table >= 0 && e.table < 3 ? e.table : 0;
+    const int m = e.material >= 0 && e.material < int(c->mats.size()) ? e.material : 0;
+    const Table& t = tab == 0 ? c->mats[size_t(m)].sp
+                              : (tab == 1 ? c->mats[size_t(m)].tr : c->mats[size_t(m)].nu);
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "Energy %.17g MeV outside %s range [%.17g, %.17g] MeV.",
+                  e.energy, what[tab], t.E.front(), t.E.back());
+    throw ValueError{buf};
+}
+
 }  // namespace
 
 extern "C" {
@@ -403,6 +495,12 @@ int m1_batch_create(m1_batch** out, int device)
         dev_alloc(&c->d_gate, 1, "hipMalloc network gate");
         host_alloc(&c->h_gate, 1, "hipHostMalloc");
         *c->h_gate = 0;
+        host_alloc(&c->h_ring, m1_batch::RING, "hipHostMalloc step status");
+        std::memset(c->h_ring, 0, m1_batch::RING * sizeof(StepStatus));
+        hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_status), c->h_ring, 0),
+                  "hipHostGetDevicePointer");
+        for (hipEvent_t& e : c->ring_ev)
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     } catch (const HipError& e) {
         g_batch_create_error = e.msg;
         delete c;
@@ -607,6 +705,10 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         hip_check(hipMemcpy(c->spatial, inflow_spatial, size_t(cols) * sizeof(double),
                             hipMemcpyHostToDevice),
                   "inflow upload");
+        upload_tables(c);
+        dev_alloc(&c->d_bc, B, "hipMalloc beamlet constants");
+        dev_alloc(&c->d_bst, B, "hipMalloc beamlet states");
+        dev_alloc(&c->d_ctr, 1, "hipMalloc step counters");
         c->have_problem = true;
     });
 }
@@ -705,9 +807,70 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // The first step evaluates every tile.
         int cur = 0;
         hip_check(hipMemsetAsync(c->d_flags[cur], 1, T, s), "tile set");
-        hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
-                                 hipMemcpyHostToDevice, s),
-                  "box flag upload");
+        // Device-side stepping: energies, CFL steps, step scalars, judge
+        // decisions and box flags stay on the device (not with recording).
+        const bool dev = c->device_steps && !c->recording;
+        const bool vacuum_aware0 =
+            n.vacuum_aware_cfl && n.tile_threshold > 0.0 && n.wave == M1_WAVE_EXACT;
+        DevStepArgs da{};
+        if (dev) {
+            std::vector<BeamletConst> bc(nb);
+            for (size_t b = 0; b < nb; ++b) {
+                const m1_beamlet& q = c->bl[b];
+                bc[b] = {q.beam.energy,  q.beam.energy_spread, q.beam.angular_spread,
+                         c->alpha[b],    E[b],                 c->outside_mats[b],
+                         tiles_of[b]};
+            }
+            hip_check(hipMemcpy(c->d_bc, bc.data(), nb * sizeof(BeamletConst),
+                                hipMemcpyHostToDevice),
+                      "beamlet constants upload");
+            hip_check(hipMemsetAsync(c->d_ctr, 0, sizeof(DevCounters), s), "counter reset");
+            hip_check(hipMemsetAsync(c->d_cfl, 0, nb * sizeof(StepCoef), s), "CFL scalar reset");
+            std::memset(c->h_ring, 0, m1_batch::RING * sizeof(StepStatus));
+            da.mats = c->d_mats;
+            da.n_mat = int(c->mats.size());
+            da.n_boxes = B;
+            da.scattering = n.scattering;
+            da.nuclear = n.nuclear ? 1 : 0;
+            da.interp = n.interp;
+            da.wave = n.wave;
+            da.ifac = ifac ? 1 : 0;
+            da.vacuum_aware = vacuum_aware0 ? 1 : 0;
+            da.cfl_box_only = n.batch_cfl_box_only ? 1 : 0;
+            da.use_model = use_model ? 1 : 0;
+            da.infer_every = c->infer_every;
+            da.max_step = n.max_step;
+            da.cfl = n.cfl;
+            da.emin = n.emin;
+            da.tile_threshold = n.tile_threshold;
+            da.dx = c->dx;
+            da.dy = c->dy;
+            da.dz = c->dz;
+            da.judge_tol = c->judge_tol;
+            da.bc = c->d_bc;
+            da.st = c->d_bst;
+            da.step = c->d_step;
+            da.coef = c->d_coef;
+            da.cfl_coef = c->d_cfl;
+            da.coef32 = c->d_coef32;
+            da.red = c->d_red;
+            da.box_mats = c->d_boxmats;
+            da.count = c->d_count;
+            da.count_full = judging ? c->d_count_full : nullptr;
+            da.sel = judging ? c->d_sel : nullptr;
+            da.jdiff = judging ? c->d_jdiff : nullptr;
+            da.jfull = judging ? c->d_jfull : nullptr;
+            da.ctr = c->d_ctr;
+            da.gate = c->d_gate;
+            da.ring = c->d_status;
+            da.ring_len = m1_batch::RING;
+            da.judge_iter = 0;
+            hip_check(launch_steps_init(da, s), "step init kernel");
+        } else {
+            hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
+                                     hipMemcpyHostToDevice, s),
+                      "box flag upload");
+        }
         {
             // Boxes with nothing to march leave the list at once.
             BatchMaskArgs ma{c->d_box,      c->d_step, c->d_tile_box, int(T), c->d_tile_max,
@@ -721,8 +884,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         const size_t nm = c->mats.size();
         // the vacuum-aware CFL needs the tile threshold (its activity level)
         // and the exact speeds (the reference applies no masks to UNIT speeds)
-        const bool vacuum_aware =
-            n.vacuum_aware_cfl && n.tile_threshold > 0.0 && n.wave == M1_WAVE_EXACT;
+        const bool vacuum_aware = vacuum_aware0;
         const bool skipping = n.tile_threshold > 0.0 && !(c->recording && c->record_full);
         int64_t iter = 0;
 
@@ -754,11 +916,15 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         a.f64_closure = n.precision == M1_FP32_BULK_F64_CLOSURE ? 1 : 0;
         // The three stages over a list; count is its host-known length (the
         // grid of the one-workgroup-per-tile launch, ignored when persistent).
+        // With device-side stepping the list length is not known on the host:
+        // count is then an estimate (a multiple of 8) and the launch strided.
+        const bool strided = dev && !persistent;
         auto stages = [&](const int* list, int* header, int count, float* dhi, float* dlo) {
             a.list = list;
             a.count = header;
             a.queue = nullptr;
             a.n_list = count;
+            a.strided = strided ? 1 : 0;
             auto persist = [&](int stage) {
                 if (persistent) {
                     a.n_list = c->persistent_grid;
@@ -816,9 +982,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // flags[k], next set flags[1 - k]); the network runs when gate (device)
         // or infer (host) says so.
         auto tail = [&](int k, const int* gate, bool infer) {
-            hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
-                                     hipMemcpyHostToDevice, s),
-                      "box flag upload");
+            if (!dev)
+                hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
+                                         hipMemcpyHostToDevice, s),
+                          "box flag upload");
             if (use_model && (gate || infer)) {
                 // a tile predicted to stay empty is left out for `horizon` steps
                 const mlp::DeviceParams prm = mlp::device_params_of(c->model);
@@ -865,9 +1032,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             c->host_ms += ms_since(t0);
         };
         auto cfl_launch = [&](int k) {
-            hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, size_t(B) * sizeof(StepCoef),
-                                     hipMemcpyHostToDevice, s),
-                      "CFL coefficient upload");
+            if (!dev)
+                hip_check(hipMemcpyAsync(c->d_cfl, c->h_cfl, size_t(B) * sizeof(StepCoef),
+                                         hipMemcpyHostToDevice, s),
+                          "CFL coefficient upload");
             hip_check(c->K->clear(c->d_red, B, s), "CFL reset");
             hip_check(c->K->clear(c->d_boxmats, B, s), "CFL reset");
             {
@@ -882,6 +1050,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 Timed t(c->prof, s, K_CFL);
                 hip_check(c->K->cfl(ca, s), "CFL kernel (batch)");
             }
+            if (dev)
+                return;  // the step kernels read the maxima on the device
             hip_check(hipMemcpyAsync(c->h_red, c->d_red, size_t(B) * sizeof(unsigned long long),
                                      hipMemcpyDeviceToHost, s),
                       "CFL read-back");
@@ -902,6 +1072,34 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             tail(k, c->d_gate, false);
             cfl_launch(1 - k);
         };
+        // Device-side stepping: the step's scalars (CFL step first) and the
+        // boundary traces; after the stages, the energies, judge decisions,
+        // box flags and next CFL scalars.
+        auto setup_dev = [&](int judge_iter) {
+            DevStepArgs x = da;
+            x.judge_iter = judge_iter;
+            {
+                Timed t(c->prof, s, K_STEP_SCALARS);
+                hip_check(launch_steps_scalars(x, s), "step scalar kernel");
+            }
+            BatchBoundaryArgs ba{c->d_box, c->d_step, B,       c->spatial,
+                                 c->ids,   c->d_coef, c->bnd, ifac ? 1 : 0};
+            Timed t(c->prof, s, K_BOUNDARY);
+            hip_check(c->K->boundary(ba, c->max_cols, s), "boundary kernel (batch)");
+        };
+        auto advance_dev = [&](int judge_iter) {
+            DevStepArgs x = da;
+            x.judge_iter = judge_iter;
+            Timed t(c->prof, s, K_STEP_ADVANCE);
+            hip_check(launch_steps_advance(x, s), "step advance kernel");
+        };
+        auto whole_step_dev = [&](int k) {
+            setup_dev(0);
+            stages(c->d_list, c->d_count, 0, c->dose_hi, c->dose_lo);
+            advance_dev(0);
+            tail(k, c->d_gate, false);
+            cfl_launch(1 - k);
+        };
         hipGraphExec_t graph[2] = {nullptr, nullptr};
         auto capture = [&](int k, int n_list) -> hipGraphExec_t {
             const hipError_t b = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -912,7 +1110,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             }
             std::string why;
             try {
-                whole_step(k, n_list);
+                if (dev)
+                    whole_step_dev(k);
+                else
+                    whole_step(k, n_list);
             } catch (const HipError& e) {
                 why = e.msg;
             }
@@ -988,6 +1189,120 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             return n_back;
         };
 
+        if (dev) {
+            // Steps are enqueued without waiting for them: the host reads the
+            // status of step k - RING (pinned memory, written by that step's
+            // scalar kernel) before enqueueing step k, to stop once no beamlet
+            // marches and to size the stage grids from that step's list
+            // length.  Steps enqueued after the last one find every beamlet
+            // done and change nothing.  Judged steps (every judge_every-th)
+            // are launched one by one, the others replayed as graphs when
+            // graphs are on.
+            const int R = m1_batch::RING;
+            auto round8 = [](long v) { return int(std::max(8L, (v + 7) / 8 * 8)); };
+            const int all_tiles = round8(long(T));
+            int est = all_tiles;  // the first step evaluates every tile
+            int prev_count = int(T);
+            cfl_launch(cur);
+            for (int64_t k = 0;; ++k) {
+                if (k >= R) {
+                    hip_check(hipEventSynchronize(c->ring_ev[k % R]), "hipEventSynchronize");
+                    const volatile StepStatus* vs = c->h_ring + k % R;
+                    const int st_active = vs->n_active, st_count = vs->count, st_err = vs->err;
+                    if (st_err != 0) {
+                        sync(c);
+                        DevCounters ctr;
+                        hip_check(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost),
+                                  "counter read-back");
+                        throw_device_error(c, ctr.err);
+                    }
+                    if (st_active == 0)
+                        break;
+                    const long mx = std::max(st_count, prev_count);
+                    prev_count = st_count;
+                    est = std::min(all_tiles, round8(mx + mx / 4 + 64));
+                    if (c->prof.on && k % 32 == 0)
+                        sync(c);
+                }
+                const bool jstep = judging && k % c->judge_every == c->judge_every - 1;
+                bool replayed = false;
+                if (graphs && !jstep) {
+                    if (!graph[cur])
+                        graph[cur] = capture(cur, 0);
+                    if (graph[cur]) {
+                        hip_check(hipGraphLaunch(graph[cur], s), "hipGraphLaunch");
+                        c->graph_steps += 1;
+                        replayed = true;
+                    } else {
+                        graphs = false;
+                    }
+                }
+                if (!replayed) {
+                    setup_dev(jstep ? 1 : 0);
+                    if (!jstep) {
+                        stages(c->d_list, c->d_count, est, c->dose_hi, c->dose_lo);
+                    } else {
+                        // as the host-stepped judge below, with the flags set by
+                        // the scalar kernel and the decision in the advance kernel
+                        const uint8_t* d_judge = c->d_sel;
+                        const uint8_t* d_save = c->d_sel + nb;
+                        slices(SLICE_SAVE, d_save);
+                        slices(SLICE_RESET, d_judge);
+                        BatchMaskArgs fa{c->d_box,   c->d_step, c->d_tile_box, int(T), c->d_tile_max,
+                                         c->d_tile_face, c->d_flags[cur], c->d_flags_full, 0.f, 0,
+                                         0, 0, 1, nullptr, nullptr, nullptr, d_judge, 1};
+                        hip_check(c->K->mask(fa, s), "judge tile set");
+                        hip_check(c->K->compact(c->d_flags_full, int(T), c->d_list_full,
+                                                c->d_count_full, s),
+                                  "judge tile list");
+                        stages(c->d_list_full, c->d_count_full, all_tiles, c->jf_hi, c->jf_lo);
+                        slices(SLICE_RESTORE, d_save);
+                        BatchZeroArgs za{c->d_box, c->d_tile_box, int(T), c->d_flags_full,
+                                         c->d_flags[cur], c->U, c->V1, c->V2, c->W1, c->W2};
+                        hip_check(c->K->zero(za, s), "judge tile reset");
+                        a.judge = d_judge;
+                        a.judge_hi = c->js_hi;
+                        a.judge_lo = c->js_lo;
+                        stages(c->d_list, c->d_count, est, c->dose_hi, c->dose_lo);
+                        a.judge = nullptr;
+                        slices(SLICE_ADD, d_judge);
+                        hip_check(hipMemsetAsync(c->d_jdiff, 0, nb * sizeof(unsigned), s),
+                                  "judge reset");
+                        hip_check(hipMemsetAsync(c->d_jfull, 0, nb * sizeof(unsigned), s),
+                                  "judge reset");
+                        BatchCompareArgs ca{c->d_box, d_judge,  B,         c->jf_hi,  c->jf_lo,
+                                            c->js_hi, c->js_lo, c->d_jdiff, c->d_jfull};
+                        hip_check(c->K->compare(ca, int(c->max_box_cells), s), "judge compare");
+                    }
+                    advance_dev(jstep ? 1 : 0);
+                    if (jstep)
+                        slices(SLICE_RESTORE, c->d_sel + 2 * nb);
+                    tail(cur, nullptr, use_model && (k + 1) % c->infer_every == 0);
+                    cfl_launch(1 - cur);
+                }
+                hip_check(hipEventRecord(c->ring_ev[k % R], s), "hipEventRecord");
+                cur = 1 - cur;
+            }
+            sync(c);
+            DevCounters ctr;
+            hip_check(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost),
+                      "counter read-back");
+            if (ctr.err.code != 0)
+                throw_device_error(c, ctr.err);
+            std::vector<BeamletState> bst(nb);
+            hip_check(hipMemcpy(bst.data(), c->d_bst, nb * sizeof(BeamletState),
+                                hipMemcpyDeviceToHost),
+                      "beamlet state read-back");
+            for (size_t b = 0; b < nb; ++b) {
+                steps[b] = bst[b].steps;
+                c->rejected[b] = uint8_t(bst[b].rejected);
+                c->rollback_to[b] = bst[b].rollback_to;
+            }
+            c->judged += int64_t(ctr.judged);
+            c->rejections += int64_t(ctr.rejections);
+            c->prof.tiles_evaluated += int64_t(ctr.tiles_evaluated);
+            c->prof.tiles_full += int64_t(ctr.tiles_full);
+        } else {
         // CFL of the first step.
         fill_cfl_host();
         cfl_launch(cur);
@@ -1168,6 +1483,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             sync(c);
             cur = 1 - cur;
         }
+        }  // host-side stepping
 
         if (dose) {
             const size_t P = size_t(c->px) * c->py * c->pz;
@@ -1312,6 +1628,59 @@ int m1_batch_graph_info(m1_batch* c, int64_t* graph_steps, const char** capture_
             *graph_steps = c->graph_steps;
         if (capture_error)
             *capture_error = c->graph_error.c_str();
+    });
+}
+
+int m1_batch_set_device_steps(m1_batch* c, int32_t enable)
+{
+    return guarded(c, [&] { c->device_steps = enable != 0; });
+}
+
+int m1_batch_physics_probe(m1_batch* c, int32_t material, int32_t kind, int32_t scattering,
+                           int32_t interpolation, const double* energies, double dE, int32_t n,
+                           double* out)
+{
+    return guarded(c, [&] {
+        if (!c->have_problem)
+            throw ValueError{"no problem bound: call m1_batch_set_problem first"};
+        if (material < 0 || material >= int(c->mats.size()))
+            throw ValueError{"material index out of range"};
+        if (kind < 0 || kind > 4)
+            throw ValueError{"kind must lie in [0, 4]"};
+        if (n < 0 || (n > 0 && (!energies || !out)))
+            throw ValueError{"energies and out must hold n values"};
+        if (n == 0)
+            return;
+        double *d_e = nullptr, *d_o = nullptr;
+        DevError* d_err = nullptr;
+        struct Free {
+            void* p[3];
+            ~Free()
+            {
+                for (void* q : p)
+                    if (q)
+                        (void)hipFree(q);
+            }
+        } guard{{nullptr, nullptr, nullptr}};
+        dev_alloc(&d_e, size_t(n), "hipMalloc probe energies");
+        guard.p[0] = d_e;
+        dev_alloc(&d_o, size_t(n), "hipMalloc probe output");
+        guard.p[1] = d_o;
+        dev_alloc(&d_err, 1, "hipMalloc probe error");
+        guard.p[2] = d_err;
+        hip_check(hipMemcpy(d_e, energies, size_t(n) * sizeof(double), hipMemcpyHostToDevice),
+                  "probe upload");
+        hip_check(hipMemsetAsync(d_err, 0, sizeof(DevError), c->stream), "probe error reset");
+        hip_check(launch_physics_probe(c->d_mats, material, kind, scattering, interpolation, d_e,
+                                       dE, n, d_o, d_err, c->stream),
+                  "physics probe kernel");
+        sync(c);
+        DevError e;
+        hip_check(hipMemcpy(&e, d_err, sizeof e, hipMemcpyDeviceToHost), "probe read-back");
+        if (e.code != 0)
+            throw_device_error(c, e);
+        hip_check(hipMemcpy(out, d_o, size_t(n) * sizeof(double), hipMemcpyDeviceToHost),
+                  "probe read-back");
     });
 }
 

@@ -286,6 +286,11 @@ struct BatchStageArgs {
     // handed out), stealing from the other slices once their own is empty.
     const int* count;
     int* queue;
+    // Estimated launch (strided set, queue null): the grid (n_list, a
+    // multiple of 8) is an estimate of the list length count[0], which the
+    // kernel reads; workgroups beyond it exit, and entries beyond the grid
+    // are taken grid-stride.
+    int strided;
     // Stage 1 / 2 stencil input formed by the previous stage (win), and the
     // one this stage forms for the next (wout); nullable.
     const float* win;

@@ -134,6 +134,9 @@ SIGNATURES = [
     ("m1_batch_set_skip_model", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
                                           C.c_int32, C.c_int32, C.c_float]),
     ("m1_batch_set_cfl_tail_floor", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_set_device_steps", C.c_int, [C.c_void_p, C.c_int32]),
+    ("m1_batch_physics_probe", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         _dp, C.c_double, C.c_int32, _dp]),
     ("m1_batch_set_skip_guard", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
     ("m1_batch_samples", C.c_int, [C.c_void_p, _fp, _fp, C.c_int64, C.POINTER(C.c_int64)]),
@@ -636,6 +639,23 @@ class Batch:
     def set_cfl_tail_floor(self, enable: bool):
         """Sub-threshold cells count with the zero state's speeds in the CFL."""
         self._check(self.lib.m1_batch_set_cfl_tail_floor(self.h, int(bool(enable))))
+
+    def set_device_steps(self, enable: bool):
+        """Energies, CFL steps, step scalars and judge decisions on the device:
+        steps are enqueued without a host round trip between them."""
+        self._check(self.lib.m1_batch_set_device_steps(self.h, int(bool(enable))))
+
+    def physics_probe(self, material: int, kind: int, energies, dE: float = 0.0,
+                      scattering: int = 2, interpolation: int = 0):
+        """Device values of the step physics of `material` at `energies`:
+        kind 0 S(E), 1 transport coefficient, 2 removal coefficient,
+        3 scattering depth over [E - dE, E], 4 nuclear depth."""
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        out = np.empty_like(e)
+        self._check(self.lib.m1_batch_physics_probe(self.h, int(material), int(kind),
+                                                    int(scattering), int(interpolation), _ptr(e),
+                                                    float(dE), int(e.size), _ptr(out)))
+        return out
 
     def set_skip_guard(self, guard_cells: int):
         """Network candidates: 4 keeps them 4 cells from any cell above the
