@@ -86,55 +86,64 @@ __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
     stage_tile<KIND, IFAC, D, true>(v);
 }
 
-// One workgroup per list entry (strided: per entry of an estimated length),
-// or (a.queue set) a persistent grid pulling
-// entries from the per-XCD queues: a workgroup of XCD x (workgroups are
-// dispatched round-robin over the XCDs) drains slice x first, keeping halo
-// re-reads of neighbouring tiles in that XCD's L2, then steals from the
-// others.  Queue counters only grow, so every workgroup ends once all slices
-// are handed out.
-template <int KIND, bool IFAC, bool D>
+// Launch forms of the stage kernels, one kernel each (a loop around the tile
+// body makes stages 1 and 2 slower than one tile per workgroup, so only the
+// forms that need a loop have one):
+//   LAUNCH_TILE      one workgroup per list entry: the list length is the
+//                    grid (a.strided == 0), or (a.strided > 0, an estimate
+//                    of it) read from count[0], workgroups past it exiting;
+//   LAUNCH_OVERFLOW  the entries past an estimate a.strided, grid-stride
+//                    (the grid and a.strided are multiples of 8, so an entry
+//                    stays on the XCD of its slot);
+//   LAUNCH_QUEUE     a persistent grid pulling entries from the per-XCD
+//                    queues: a workgroup of XCD x (workgroups are dispatched
+//                    round-robin over the XCDs) drains slice x first,
+//                    keeping halo re-reads of neighbouring tiles in that
+//                    XCD's L2, then steals from the others.  Queue counters
+//                    only grow, so every workgroup ends once all slices are
+//                    handed out.
+enum StageLaunch : int { LAUNCH_TILE = 0, LAUNCH_OVERFLOW = 1, LAUNCH_QUEUE = 2 };
+
+template <int KIND, bool IFAC, bool D, int L>
 __global__ void __launch_bounds__(NT, M1_STAGE_WAVES) stage_kernel(BatchStageArgs a)
 {
-    if (!a.queue) {
-        if (!a.strided) {
-            run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(int(blockIdx.x), a.n_list)]);
-            return;
-        }
-        // Grid of an estimated list length (the length is read here): slot i
-        // as in the exact launch while the grid covers the list, and the
-        // workgroups take further slots grid-stride when it does not (the
-        // grid is a multiple of 8, so a slot stays on its XCD).
-        const int n = *a.count;
-        for (int i = int(blockIdx.x); i < n; i += int(gridDim.x)) {
+    if constexpr (L == LAUNCH_TILE) {
+        const int n = a.strided ? *a.count : a.n_list;
+        const int i = int(blockIdx.x);
+        if (i < n)
             run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(i, n)]);
-            if (i + int(gridDim.x) < n)
+    } else if constexpr (L == LAUNCH_OVERFLOW) {
+        const int n = *a.count;
+        const int step = int(gridDim.x);
+        for (int j = a.strided + int(blockIdx.x); j < n; j += step) {
+            run_tile<KIND, IFAC, D>(a, a.list[xcd_slot(j, n)]);
+            if (j + step < n)
                 __syncthreads();  // the next tile reuses LDS
         }
-        return;
-    }
-    __shared__ int s_slot;
-    const int n = *a.count;
-    const int q = n / XCD_QUEUES, r = n % XCD_QUEUES;
-    const int home = int(blockIdx.x) % XCD_QUEUES;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            int got = -1;
-            for (int k = 0; k < XCD_QUEUES && got < 0; ++k) {
-                const int x = (home + k) % XCD_QUEUES;
-                const int len = q + (x < r ? 1 : 0);
-                const int s = atomicAdd(a.queue + x, 1);
-                if (s < len)
-                    got = x * q + min(x, r) + s;
+    } else {
+        __shared__ int s_slot;
+        const int n = *a.count;
+        const int q = n / XCD_QUEUES, r = n % XCD_QUEUES;
+        const int home = int(blockIdx.x) % XCD_QUEUES;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                int got = -1;
+                for (int k = 0; k < XCD_QUEUES && got < 0; ++k) {
+                    const int x = (home + k) % XCD_QUEUES;
+                    const int len = q + (x < r ? 1 : 0);
+                    const int s = atomicAdd(a.queue + x, 1);
+                    if (s < len)
+                        got = x * q + min(x, r) + s;
+                }
+                s_slot = got;
             }
-            s_slot = got;
+            __syncthreads();
+            const int slot = s_slot;
+            if (slot < 0)
+                break;
+            run_tile<KIND, IFAC, D>(a, a.list[slot]);
+            __syncthreads();  // the next tile reuses LDS and s_slot
         }
-        __syncthreads();
-        const int slot = s_slot;
-        if (slot < 0)
-            break;
-        run_tile<KIND, IFAC, D>(a, a.list[slot]);
-        __syncthreads();  // the next tile reuses LDS and s_slot
     }
 }
 
@@ -827,25 +836,74 @@ int stage_blocks_per_cu()
 {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&stage_kernel<STAGE2, true, false>), NT, 0) !=
+            &n, reinterpret_cast<const void*>(&stage_kernel<STAGE2, true, false, LAUNCH_TILE>),
+            NT, 0) !=
             hipSuccess ||
         n < 1)
         n = 1;
     return n;
 }
 
+template <int KIND, bool IFAC, bool D>
+int stage_form_of(const void* func)
+{
+    if (func == reinterpret_cast<const void*>(&stage_kernel<KIND, IFAC, D, LAUNCH_TILE>))
+        return LAUNCH_TILE;
+    if (func == reinterpret_cast<const void*>(&stage_kernel<KIND, IFAC, D, LAUNCH_OVERFLOW>))
+        return LAUNCH_OVERFLOW;
+    if (func == reinterpret_cast<const void*>(&stage_kernel<KIND, IFAC, D, LAUNCH_QUEUE>))
+        return LAUNCH_QUEUE;
+    return -1;
+}
+
+int stage_form(const void* func)
+{
+    int f = -1;
+    for (int d = 0; d < 2 && f < 0; ++d) {
+        const bool dd = d != 0;
+        f = dd ? stage_form_of<STAGE0, false, true>(func) : stage_form_of<STAGE0, false, false>(func);
+        if (f < 0)
+            f = dd ? stage_form_of<STAGE1, false, true>(func) : stage_form_of<STAGE1, false, false>(func);
+        if (f < 0)
+            f = dd ? stage_form_of<STAGE1, true, true>(func) : stage_form_of<STAGE1, true, false>(func);
+        if (f < 0)
+            f = dd ? stage_form_of<STAGE2, false, true>(func) : stage_form_of<STAGE2, false, false>(func);
+        if (f < 0)
+            f = dd ? stage_form_of<STAGE2, true, true>(func) : stage_form_of<STAGE2, true, false>(func);
+    }
+    return f;
+}
+
 hipError_t launch_stage(int kind, bool ifac, const BatchStageArgs& a, hipStream_t s)
 {
-    // persistent: n_list is the grid (workgroups resident at once)
+    // persistent (queue set): n_list is the grid (workgroups resident at
+    // once); estimated (strided > 0): strided one-tile workgroups, then
+    // n_list - strided overflow workgroups; otherwise n_list workgroups
     if (a.n_list <= 0)
         return hipSuccess;
-    const dim3 grid(unsigned(a.n_list)), block(NT);
+    const int form = a.queue ? LAUNCH_QUEUE : LAUNCH_TILE;
+    const unsigned main_grid = unsigned(a.queue || !a.strided ? a.n_list : a.strided);
+    const unsigned over_grid = unsigned(a.queue || !a.strided ? 0 : a.n_list - a.strided);
+    const dim3 block(NT);
+#define M1_LAUNCH_FORM(K, I, DD, L, G)                                                    \
+    hipLaunchKernelGGL((stage_kernel<K, I, DD, L>), dim3(G), block, 0, s, a)
 #define M1_LAUNCH(K, I)                                                                   \
     do {                                                                                  \
-        if (a.f64_closure)                                                                \
-            hipLaunchKernelGGL((stage_kernel<K, I, true>), grid, block, 0, s, a);         \
-        else                                                                              \
-            hipLaunchKernelGGL((stage_kernel<K, I, false>), grid, block, 0, s, a);        \
+        if (a.f64_closure) {                                                              \
+            if (form == LAUNCH_QUEUE)                                                     \
+                M1_LAUNCH_FORM(K, I, true, LAUNCH_QUEUE, main_grid);                      \
+            else                                                                          \
+                M1_LAUNCH_FORM(K, I, true, LAUNCH_TILE, main_grid);                       \
+            if (over_grid)                                                                \
+                M1_LAUNCH_FORM(K, I, true, LAUNCH_OVERFLOW, over_grid);                   \
+        } else {                                                                          \
+            if (form == LAUNCH_QUEUE)                                                     \
+                M1_LAUNCH_FORM(K, I, false, LAUNCH_QUEUE, main_grid);                     \
+            else                                                                          \
+                M1_LAUNCH_FORM(K, I, false, LAUNCH_TILE, main_grid);                      \
+            if (over_grid)                                                                \
+                M1_LAUNCH_FORM(K, I, false, LAUNCH_OVERFLOW, over_grid);                  \
+        }                                                                                 \
     } while (0)
     switch (kind) {
     case STAGE0:
@@ -867,6 +925,7 @@ hipError_t launch_stage(int kind, bool ifac, const BatchStageArgs& a, hipStream_
         return hipErrorInvalidValue;
     }
 #undef M1_LAUNCH
+#undef M1_LAUNCH_FORM
     return hipGetLastError();
 }
 
@@ -1019,7 +1078,7 @@ const BatchKernels& M1_BATCH_TABLE()
                                 &launch_mask,   &launch_compact, &launch_zero,
                                 &launch_scatter_dose, &launch_box_dose, &launch_features,
                                 &launch_labels, &launch_compare, &launch_slices,
-                                &launch_clear,  &stage_blocks_per_cu};
+                                &launch_clear,  &stage_blocks_per_cu, &stage_form};
     return k;
 }
 }  // namespace m1

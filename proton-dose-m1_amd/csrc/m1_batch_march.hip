@@ -895,7 +895,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // as separate launches, and so does every step while kernels are
         // timed (the per-launch event pairs stay outside graphs).
         bool graphs = c->use_graphs && !c->recording && !c->prof.on;
-        const bool persistent = c->persistent || graphs;
+        // (host-stepped graphs replay the persistent launch; device-stepped
+        // graphs the estimated one, its grids set before each replay)
+        const bool persistent = c->persistent || (graphs && !dev);
 
         // Stage arguments common to every launch of this march.
         BatchStageArgs a{};
@@ -919,12 +921,15 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // With device-side stepping the list length is not known on the host:
         // count is then an estimate (a multiple of 8) and the launch strided.
         const bool strided = dev && !persistent;
+        auto overflow_wgs = [](int count) { return std::max(64, count / 256 * 8); };
         auto stages = [&](const int* list, int* header, int count, float* dhi, float* dlo) {
             a.list = list;
             a.count = header;
             a.queue = nullptr;
-            a.n_list = count;
-            a.strided = strided ? 1 : 0;
+            // estimated launch: `count` one-tile workgroups and a few more for
+            // entries past the estimate
+            a.n_list = strided ? count + overflow_wgs(count) : count;
+            a.strided = strided ? count : 0;
             auto persist = [&](int stage) {
                 if (persistent) {
                     a.n_list = c->persistent_grid;
@@ -1093,14 +1098,71 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             Timed t(c->prof, s, K_STEP_ADVANCE);
             hip_check(launch_steps_advance(x, s), "step advance kernel");
         };
+        int graph_est = 8;  // the estimate a device-stepped graph is captured with
         auto whole_step_dev = [&](int k) {
             setup_dev(0);
-            stages(c->d_list, c->d_count, 0, c->dose_hi, c->dose_lo);
+            stages(c->d_list, c->d_count, graph_est, c->dose_hi, c->dose_lo);
             advance_dev(0);
             tail(k, c->d_gate, false);
             cfl_launch(1 - k);
         };
         hipGraphExec_t graph[2] = {nullptr, nullptr};
+        // Device-stepped graphs: the captured graphs and their stage-kernel
+        // nodes, whose grids and estimates are set before each replay.
+        struct StageNode {
+            hipGraphNode_t node;
+            int form;
+            hipKernelNodeParams p;
+            BatchStageArgs args;
+        };
+        hipGraph_t graph_src[2] = {nullptr, nullptr};
+        std::vector<StageNode> stage_nodes[2];
+        auto find_stage_nodes = [&](int k, hipGraph_t g) -> bool {
+            size_t n = 0;
+            if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess)
+                return false;
+            std::vector<hipGraphNode_t> nodes(n);
+            if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess)
+                return false;
+            stage_nodes[k].clear();
+            for (hipGraphNode_t nd : nodes) {
+                hipGraphNodeType t;
+                if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel)
+                    continue;
+                hipKernelNodeParams p{};
+                if (hipGraphKernelNodeGetParams(nd, &p) != hipSuccess)
+                    return false;
+                const int form = c->K->stage_form(p.func);
+                if (form < 0)
+                    continue;
+                if (form == 2 || !p.kernelParams || !p.kernelParams[0])
+                    return false;
+                StageNode sn{nd, form, p, {}};
+                std::memcpy(&sn.args, p.kernelParams[0], sizeof(BatchStageArgs));
+                stage_nodes[k].push_back(sn);
+            }
+            // every stage as a one-tile launch and its overflow launch
+            int forms[2] = {0, 0};
+            for (const StageNode& sn : stage_nodes[k])
+                forms[sn.form] += 1;
+            return forms[0] == 3 && forms[1] == 3;
+        };
+        auto set_graph_grids = [&](int k, int count) -> bool {
+            for (StageNode& sn : stage_nodes[k]) {
+                BatchStageArgs x = sn.args;
+                const int over = overflow_wgs(count);
+                x.strided = count;
+                x.n_list = count + over;
+                hipKernelNodeParams p = sn.p;
+                p.gridDim = dim3(unsigned(sn.form == 1 ? over : count));
+                void* args[1] = {&x};
+                p.kernelParams = args;
+                p.extra = nullptr;
+                if (hipGraphExecKernelNodeSetParams(graph[k], sn.node, &p) != hipSuccess)
+                    return false;
+            }
+            return true;
+        };
         auto capture = [&](int k, int n_list) -> hipGraphExec_t {
             const hipError_t b = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             if (b != hipSuccess) {
@@ -1122,9 +1184,16 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             hipGraphExec_t ex = nullptr;
             if (why.empty() && e == hipSuccess && g)
                 e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            if (g)
+            if (why.empty() && e == hipSuccess && ex && dev && !find_stage_nodes(k, g))
+                why = "stage-kernel nodes of the captured step not found";
+            if (g && dev && why.empty() && e == hipSuccess && ex)
+                graph_src[k] = g;  // kept: its nodes address the replay's updates
+            else if (g)
                 (void)hipGraphDestroy(g);
             if (!why.empty() || e != hipSuccess || !ex) {
+                if (ex)
+                    (void)hipGraphExecDestroy(ex);
+                ex = nullptr;
                 c->graph_error = why.empty() ? std::string(hipGetErrorString(e)) : why;
                 (void)hipGetLastError();
                 return nullptr;
@@ -1133,13 +1202,17 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         };
         struct GraphGuard {
             hipGraphExec_t* g;
+            hipGraph_t* src;
             ~GraphGuard()
             {
-                for (int k = 0; k < 2; ++k)
+                for (int k = 0; k < 2; ++k) {
                     if (g[k])
                         (void)hipGraphExecDestroy(g[k]);
+                    if (src[k])
+                        (void)hipGraphDestroy(src[k]);
+                }
             }
-        } graph_guard{graph};
+        } graph_guard{graph, graph_src};
         c->graph_steps = 0;
 
         // Energies after a step and next step's box flags; the judge's
@@ -1227,9 +1300,14 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 const bool jstep = judging && k % c->judge_every == c->judge_every - 1;
                 bool replayed = false;
                 if (graphs && !jstep) {
+                    graph_est = est;
                     if (!graph[cur])
                         graph[cur] = capture(cur, 0);
-                    if (graph[cur]) {
+                    if (graph[cur] && !set_graph_grids(cur, est)) {
+                        c->graph_error = "setting the stage grids of a graph failed";
+                        (void)hipGetLastError();
+                        graphs = false;
+                    } else if (graph[cur]) {
                         hip_check(hipGraphLaunch(graph[cur], s), "hipGraphLaunch");
                         c->graph_steps += 1;
                         replayed = true;

@@ -286,10 +286,11 @@ struct BatchStageArgs {
     // handed out), stealing from the other slices once their own is empty.
     const int* count;
     int* queue;
-    // Estimated launch (strided set, queue null): the grid (n_list, a
-    // multiple of 8) is an estimate of the list length count[0], which the
-    // kernel reads; workgroups beyond it exit, and entries beyond the grid
-    // are taken grid-stride.
+    // Estimated launch (strided > 0, queue null): strided (a multiple of 8)
+    // is an estimate of the list length count[0], which the kernel reads;
+    // workgroup i < strided takes slot i (or exits past the list), and the
+    // n_list - strided workgroups after them take the entries past the
+    // estimate, grid-stride.
     int strided;
     // Stage 1 / 2 stencil input formed by the previous stage (win), and the
     // one this stage forms for the next (wout); nullable.
@@ -483,6 +484,9 @@ struct BatchKernels {
     hipError_t (*clear)(unsigned long long* p, int n, hipStream_t s);
     // stage-kernel workgroups resident per CU (occupancy)
     int (*blocks_per_cu)();
+    // launch form of a stage kernel (0 one tile per workgroup, 1 overflow,
+    // 2 queue), -1 when func is not one of this table's stage kernels
+    int (*stage_form)(const void* func);
 };
 
 // Tile shapes (x, y, z cells; y * z = 256 threads, x * y <= 256) the batched

@@ -3,7 +3,11 @@
 The batched kernels are compiled for several workgroup tile shapes
 (`_native.batch_tile_shapes()`, from M1_BATCH_SHAPES in csrc/m1_device.h) and
 can be launched three ways: one workgroup per tile, a persistent grid pulling
-tiles from per-XCD queues, or plain steps replayed from HIP graphs.  The tile
+tiles from per-XCD queues, or plain steps replayed from HIP graphs; and the
+energy steps can be set up on the host (one round trip per step) or on the
+device ("device_*" modes: per-beamlet CFL steps, step scalars and judge
+decisions in kernels, steps enqueued without waiting; the per-tile launch is
+then sized from an estimate of the list length).  The tile
 shape sets the skip granularity (finer tiles skip more vacuum but re-read
 more halo cells per cell) and the register and LDS footprint of the stage
 kernels, so the best one depends on the beamlets and the phantom.
@@ -25,7 +29,7 @@ import numpy as np
 
 from . import _native
 
-MODES = ("per_tile", "persistent", "graphs")
+MODES = ("per_tile", "persistent", "graphs", "device", "device_persistent", "device_graphs")
 
 
 @dataclass(frozen=True)
@@ -44,8 +48,9 @@ class Choice:
 def set_mode(handle: _native.Batch, mode: str):
     if mode not in MODES:
         raise ValueError(f"mode must be one of {MODES}")
-    handle.set_persistent(mode == "persistent")
-    handle.set_graphs(mode == "graphs")
+    handle.set_device_steps(mode.startswith("device"))
+    handle.set_persistent(mode.endswith("persistent"))
+    handle.set_graphs(mode.endswith("graphs"))
 
 
 def sample_beamlets(beamlets, n):
