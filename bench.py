@@ -347,7 +347,7 @@ def run_batch(args):
 
         t0 = time.perf_counter()
         choice = autotune(phantom, beamlets, cfg, sample=args.autotune_sample,
-                          modes=("per_tile", "persistent"),
+                          modes=("per_tile", "device", "device_graphs"),
                           log=lambda row: print(json.dumps({"autotune": row}), file=sys.stderr,
                                                 flush=True),
                           setup=lambda h: tileskip.enable(h, model, net_opts))
@@ -362,6 +362,10 @@ def run_batch(args):
                                               (args.tile_shape or 0)))
     if tuned:
         choice.apply(handle)
+    else:
+        from proton_solver.autotune import set_mode
+
+        set_mode(handle, args.launch)
     setup_s = time.perf_counter() - t0
     num = _native.numerics_struct(cfg)
     tileskip.enable(handle, model, net_opts)
@@ -555,7 +559,7 @@ def run_batch(args):
             "n_beamlets": B,
             "precision": args.batch_precision,
             "tile_shape": list(handle.tile_shape),
-            "launch": tuned["mode"] if tuned else "per_tile",
+            "launch": tuned["mode"] if tuned else args.launch,
             "box_cells_total": int(box_cells.sum()),
             "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
                                          "max": int(steps.max())},
@@ -603,7 +607,9 @@ def run_combined(args):
     num = _native.numerics_struct(cfg, _native.FP32_BULK_F64_CLOSURE)
     use_net = tileskip.WEIGHTS_FILE.exists() and args.skip_network != "off"
     tileskip.enable(handle, tileskip.load_model(0) if use_net else None, tileskip.SkipOptions())
-    handle.set_graphs(True)
+    from proton_solver.autotune import set_mode
+
+    set_mode(handle, "device_graphs")  # device-side stepping, graph-replayed steps
     for _ in range(args.warmup):
         handle.march(num, want_dose=False)
     t0 = time.perf_counter()
@@ -614,7 +620,7 @@ def run_combined(args):
     B = len(chosen)
     doses = [handle.beamlet_dose(b) for b in range(B)]
     # one profiled march (launched step by step) for the dominant kernel
-    handle.set_graphs(False)
+    set_mode(handle, "device")
     handle.set_profiling(True)
     handle.reset_stats()
     handle.march(num, want_dose=False)
@@ -864,6 +870,12 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="default tile shape and launch instead of the autotuner's choice")
     ap.add_argument("--autotune-sample", type=int, default=24)
+    ap.add_argument("--launch", default="device_graphs",
+                    choices=["per_tile", "persistent", "graphs", "device", "device_persistent",
+                             "device_graphs"],
+                    help="launch of the batch when the autotuner does not choose it: "
+                         "host- or device-stepped energy steps, one workgroup per tile, a "
+                         "persistent grid or graph replay")
     ap.add_argument("--tile-shape", type=int, default=None,
                     help="index of the batched kernels' tile shape (no autotuning)")
     ap.add_argument("--profile-calibration", action="store_true",
