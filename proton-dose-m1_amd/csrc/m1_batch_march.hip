@@ -1099,6 +1099,7 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             hip_check(launch_steps_advance(x, s), "step advance kernel");
         };
         int graph_est = 8;  // the estimate a device-stepped graph is captured with
+        int graph_set[2] = {0, 0};  // the estimate each graph's stage grids hold
         auto whole_step_dev = [&](int k) {
             setup_dev(0);
             stages(c->d_list, c->d_count, graph_est, c->dose_hi, c->dose_lo);
@@ -1300,10 +1301,22 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 const bool jstep = judging && k % c->judge_every == c->judge_every - 1;
                 bool replayed = false;
                 if (graphs && !jstep) {
-                    graph_est = est;
-                    if (!graph[cur])
+                    // the estimate rounded up to 1/16 - 1/8 of itself, so that most
+                    // replays keep the grids of the last one
+                    int q = 64;
+                    while (q * 16 < est)
+                        q *= 2;
+                    const int est_q = std::min(all_tiles, (est + q - 1) / q * q);
+                    graph_est = est_q;
+                    bool set_ok = true;
+                    if (!graph[cur]) {
                         graph[cur] = capture(cur, 0);
-                    if (graph[cur] && !set_graph_grids(cur, est)) {
+                        graph_set[cur] = est_q;
+                    } else if (graph_set[cur] != est_q) {
+                        set_ok = set_graph_grids(cur, est_q);
+                        graph_set[cur] = est_q;
+                    }
+                    if (graph[cur] && !set_ok) {
                         c->graph_error = "setting the stage grids of a graph failed";
                         (void)hipGetLastError();
                         graphs = false;
