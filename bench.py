@@ -81,14 +81,14 @@ STEP_BYTES_FP32 = sum(STAGE_BYTES_FP32.values())
 # TRAFFIC_PROFILE); shapes without a measurement report traffic null.
 TRAFFIC_PROFILE = "profiles/stage_kernels_batch_pmc_traffic.md"
 PMC_BYTES_PER_CELL_FP32 = {
-    ((8, 16, 16), "fp32_bulk"): {"stage0": 101.6, "stage1": 150.8, "stage2": 152.6},
+    ((8, 16, 16), "fp32_bulk"): {"stage0": 101.8, "stage1": 151.3, "stage2": 153.3},
 }
 # Fraction of the VALU issue slots the batched stage kernels use
 # (rocprofv3 SQ_INSTS_VALU x 4 cycles / (kernel time x 2.4 GHz x 1024
 # SIMDs), VALU_PROFILE): the bound of these kernels.
 VALU_PROFILE = "profiles/stage_kernels_batch_valu.md"
 VALU_ISSUE_FRAC_FP32 = {
-    ((8, 16, 16), "fp32_bulk"): {"stage0": 0.80, "stage1": 0.72, "stage2": 0.70},
+    ((8, 16, 16), "fp32_bulk"): {"stage0": 0.81, "stage1": 0.73, "stage2": 0.71},
 }
 
 
