@@ -773,15 +773,24 @@ __global__ void __launch_bounds__(256)
 // Tiles that leave the evaluated set become exact vacuum (U = V1 = V2 = 0).
 __device__ __forceinline__ void zero_tile(const BatchZeroArgs& a, int t);
 
-// A grid of at most ZERO_GRID workgroups walks the tiles; each zeroes the
-// leaving ones it meets.
-constexpr int ZERO_GRID = 4096;
-
+// Workgroup g checks tiles [g NT, (g + 1) NT) at once (one lane each), then
+// zeroes the leaving ones among them.  (A capped grid walking the tiles one
+// flag load after another was latency-bound: ~150 us per step on the
+// 256-beamlet batch.)
 __global__ void __launch_bounds__(NT) zero_kernel(BatchZeroArgs a)
 {
-    for (int t = blockIdx.x; t < a.n_tiles; t += gridDim.x)
-        if (a.computed[t] && !a.next[t])
-            zero_tile(a, t);
+    __shared__ int s_list[NT];
+    __shared__ int s_n;
+    if (threadIdx.x == 0)
+        s_n = 0;
+    __syncthreads();
+    const int t = int(blockIdx.x) * NT + int(threadIdx.x);
+    if (t < a.n_tiles && a.computed[t] && !a.next[t])
+        s_list[atomicAdd(&s_n, 1)] = t;
+    __syncthreads();
+    const int n = s_n;
+    for (int k = 0; k < n; ++k)
+        zero_tile(a, s_list[k]);
 }
 
 __device__ __forceinline__ void zero_tile(const BatchZeroArgs& a, int t)
@@ -997,8 +1006,7 @@ hipError_t launch_zero(const BatchZeroArgs& a, hipStream_t s)
 {
     if (a.n_tiles <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min(a.n_tiles, ZERO_GRID))), dim3(NT), 0,
-                       s, a);
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned((a.n_tiles + NT - 1) / NT)), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
