@@ -930,6 +930,9 @@ hipError_t launch_stage(int kind, bool ifac, const BatchStageArgs& a, hipStream_
         else
             M1_LAUNCH(STAGE2, false);
         break;
+    case STAGE_RELAX:
+        M1_LAUNCH(STAGE_RELAX, false);
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -1069,6 +1072,58 @@ hipError_t launch_slices(const BatchSliceArgs& a, int max_cells, hipStream_t s)
     return hipGetLastError();
 }
 
+// Live tiles of active boxes holding a quasi-steady material.
+__global__ void air_mask_kernel(BatchAirArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_tiles)
+        return;
+    a.air[t] = a.live[t] && a.step[a.tile_box[t]].active && (a.tile_mats[t] & a.qs_mask) ? 1 : 0;
+}
+
+// The listed tiles' quasi-steady cells take the relaxed state (from -> to).
+__global__ void __launch_bounds__(NT) air_commit_kernel(BatchAirArgs a)
+{
+    const int n = *a.count;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int t = a.list[k];
+        const int b = a.tile_box[t];
+        const BoxDesc d = a.box[b];
+        const TileCoord tc = tile_coord(d, b, t);
+        const long N = long(d.nx) * d.ny * d.nz;
+        const int z = tc.tz * TZ + threadIdx.x % TZ;
+        const int y = tc.ty * TY + threadIdx.x / TZ;
+        if (z >= d.nz || y >= d.ny)
+            continue;
+        const long off4 = 4 * d.cell_off;
+        for (int x = tc.tx * TX; x < min(tc.tx * TX + TX, d.nx); ++x) {
+            const long i = (long(x) * d.ny + y) * d.nz + z;
+            if (!(a.qs_mask >> a.ids[d.cell_off + i] & 1u))
+                continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                a.to[off4 + i + c * N] = a.from[off4 + i + c * N];
+        }
+    }
+}
+
+hipError_t launch_air_mask(const BatchAirArgs& a, hipStream_t s)
+{
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(air_mask_kernel, dim3(unsigned((a.n_tiles + 255) / 256)), dim3(256), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
+hipError_t launch_air_commit(const BatchAirArgs& a, int grid, hipStream_t s)
+{
+    if (grid <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(air_commit_kernel, dim3(unsigned(grid)), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_box_dose(const float* hi, const float* lo, long n, double* out, hipStream_t s)
 {
     hipLaunchKernelGGL(box_dose_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, hi, lo,
@@ -1086,7 +1141,8 @@ const BatchKernels& M1_BATCH_TABLE()
                                 &launch_mask,   &launch_compact, &launch_zero,
                                 &launch_scatter_dose, &launch_box_dose, &launch_features,
                                 &launch_labels, &launch_compare, &launch_slices,
-                                &launch_clear,  &stage_blocks_per_cu, &stage_form};
+                                &launch_clear,  &stage_blocks_per_cu, &stage_form,
+                                &launch_air_mask, &launch_air_commit};
     return k;
 }
 }  // namespace m1

@@ -236,6 +236,16 @@ struct m1_batch {
     float *d_rx = nullptr, *d_ry = nullptr;
     int* d_rcount = nullptr;
 
+    // quasi-steady materials (m1_batch_set_quasi_steady)
+    uint32_t qs_mask = 0;
+    int qs_iters = 0;
+    double qs_cfl = 0.5;
+    StepCoef32* d_relax32 = nullptr;  // per box: the relaxation's coefficients
+    uint8_t* d_air = nullptr;         // per tile: quasi-steady tiles to relax
+    int* d_air_list = nullptr;
+    int* d_air_count = nullptr;
+    float* d_relax_tmp = nullptr;     // 4 x cells: a relaxation sweep's result
+    long tiles_with_mat[MAXMAT] = {};  // tiles of the boxes holding material m
     // device-side energy stepping (m1_batch_set_device_steps)
     bool device_steps = false;
     DevMaterial* d_mats = nullptr;  // material tables on the device
@@ -305,9 +315,15 @@ void m1_batch::free_problem()
     for (void* p : dev)
         if (p)
             (void)hipFree(p);
-    for (void* p : {(void*)d_mats, (void*)d_tables, (void*)d_bc, (void*)d_bst, (void*)d_ctr})
+    for (void* p : {(void*)d_mats, (void*)d_tables, (void*)d_bc, (void*)d_bst, (void*)d_ctr,
+                    (void*)d_relax32, (void*)d_air, (void*)d_air_list, (void*)d_air_count,
+                    (void*)d_relax_tmp})
         if (p)
             (void)hipFree(p);
+    d_relax32 = nullptr;
+    d_relax_tmp = nullptr;
+    d_air = nullptr;
+    d_air_list = d_air_count = nullptr;
     d_mats = nullptr;
     d_tables = nullptr;
     d_bc = nullptr;
@@ -629,6 +645,10 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
             if (tile_cells[t] > 0)
                 tile_rho[t] /= float(tile_cells[t]);
         }
+        long with_mat[MAXMAT] = {};
+        for (uint8_t tm : tile_mats)
+            for (int m = 0; m < MAXMAT; ++m)
+                with_mat[m] += (tm >> m) & 1;
 
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         c->free_problem();
@@ -648,6 +668,7 @@ int m1_batch_set_problem(m1_batch* c, int32_t nx, int32_t ny, int32_t nz, double
         c->total_tiles = int(tiles);
         c->max_cols = max_cols;
         c->max_box_cells = max_cells;
+        std::copy(with_mat, with_mat + MAXMAT, c->tiles_with_mat);
         const size_t B = size_t(n_beamlets), T = size_t(tiles), N = size_t(cells);
         dev_alloc(&c->d_box, B, "hipMalloc boxes");
         dev_alloc(&c->d_step, B, "hipMalloc box flags");
@@ -810,6 +831,13 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // Device-side stepping: energies, CFL steps, step scalars, judge
         // decisions and box flags stay on the device (not with recording).
         const bool dev = c->device_steps && !c->recording;
+        const bool qs = c->qs_mask != 0 && c->qs_iters > 0;
+        if (qs && !dev)
+            throw NotImplemented{"quasi-steady materials need device-side stepping "
+                                 "(m1_batch_set_device_steps)"};
+        if (qs && c->model)
+            throw NotImplemented{"quasi-steady materials and the tile-skip network are not "
+                                 "combined"};
         const bool vacuum_aware0 =
             n.vacuum_aware_cfl && n.tile_threshold > 0.0 && n.wave == M1_WAVE_EXACT;
         DevStepArgs da{};
@@ -865,6 +893,33 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             da.ring = c->d_status;
             da.ring_len = m1_batch::RING;
             da.judge_iter = 0;
+            da.qs_mask = qs ? c->qs_mask : 0u;
+            if (qs) {
+                // relaxation coefficients: the pseudo-time CFL number on the
+                // quasi-steady materials, the box's WENO and speed floors
+                if (!c->d_relax32) {
+                    dev_alloc(&c->d_relax32, nb, "hipMalloc relaxation coefficients");
+                    dev_alloc(&c->d_air, T, "hipMalloc quasi-steady tiles");
+                    dev_alloc(&c->d_air_list, T, "hipMalloc quasi-steady tile list");
+                    dev_alloc(&c->d_air_count, LIST_HEADER_ALLOC, "hipMalloc tile count");
+                    dev_alloc(&c->d_relax_tmp, 4 * N, "hipMalloc relaxation state");
+                }
+                std::vector<StepCoef32> rc(nb);
+                for (size_t b = 0; b < nb; ++b) {
+                    StepCoef32& f = rc[b];
+                    std::memset(&f, 0, sizeof f);
+                    const double al = c->alpha[b];
+                    f.weno_eps = float(std::max(1.0e-12 / (al * al), 1.0e-37));
+                    f.eig_eps = 1.0e-10 / al;
+                    for (size_t m = 0; m < c->mats.size(); ++m) {
+                        f.S0[m] = f.S1[m] = f.S2[m] = 1.f;
+                        f.dEm[m] = (c->qs_mask >> m & 1u) ? float(c->qs_cfl) : 0.f;
+                    }
+                }
+                hip_check(hipMemcpy(c->d_relax32, rc.data(), nb * sizeof(StepCoef32),
+                                    hipMemcpyHostToDevice),
+                          "relaxation coefficient upload");
+            }
             hip_check(launch_steps_init(da, s), "step init kernel");
         } else {
             hip_check(hipMemcpyAsync(c->d_step, c->h_step, size_t(B) * sizeof(BoxStep),
@@ -894,7 +949,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // list length read on the device).  Judged and recording steps run
         // as separate launches, and so does every step while kernels are
         // timed (the per-launch event pairs stay outside graphs).
-        bool graphs = c->use_graphs && !c->recording && !c->prof.on;
+        bool graphs = c->use_graphs && !c->recording && !c->prof.on &&
+                      !(c->qs_mask != 0 && c->qs_iters > 0);
         // (host-stepped graphs replay the persistent launch; device-stepped
         // graphs the estimated one, its grids set before each replay)
         const bool persistent = c->persistent || (graphs && !dev);
@@ -922,6 +978,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // count is then an estimate (a multiple of 8) and the launch strided.
         const bool strided = dev && !persistent;
         auto overflow_wgs = [](int count) { return std::max(64, count / 256 * 8); };
+        // called after the launches of stages 0 and 1 (the quasi-steady
+        // relaxation of the next stage's stencil input)
+        std::function<void(int)> between_stages;
         auto stages = [&](const int* list, int* header, int count, float* dhi, float* dlo) {
             a.list = list;
             a.count = header;
@@ -947,6 +1006,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 Timed t(c->prof, s, K_STAGE0);
                 hip_check(c->K->stage(STAGE0, false, a, s), "stage 0 kernel (batch)");
             }
+            if (between_stages)
+                between_stages(0);
             {
                 persist(1);
                 a.src = c->V1;
@@ -957,6 +1018,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                 Timed t(c->prof, s, K_STAGE1);
                 hip_check(c->K->stage(STAGE1, ifac, a, s), "stage 1 kernel (batch)");
             }
+            if (between_stages)
+                between_stages(1);
             {
                 persist(2);
                 a.src = c->V2;
@@ -1097,6 +1160,51 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             x.judge_iter = judge_iter;
             Timed t(c->prof, s, K_STEP_ADVANCE);
             hip_check(launch_steps_advance(x, s), "step advance kernel");
+        };
+        // Quasi-steady materials: after each stage, qs_iters Jacobi sweeps of
+        // a pseudo-time step towards div F = 0 on their cells of the next
+        // stage's stencil input (W1, W2, then U; the other cells fixed), over
+        // the live tiles holding them: a sweep's result goes to a scratch
+        // state, then back.  The quasi-steady cells thus follow the other
+        // cells at every stage (no lag of a step).
+        long air_bound = 0;
+        for (int m = 0; m < MAXMAT; ++m)
+            if (c->qs_mask >> m & 1u)
+                air_bound += c->tiles_with_mat[m];
+        const int air_est = int(std::max(8L, std::min(long(T), (air_bound + 7) / 8 * 8)));
+        auto relax_air = [&](int k, float* state, int bnd_stage, bool build_list) {
+            BatchAirArgs aa{c->d_box, c->d_step, c->d_tile_box, int(T), c->d_flags[k],
+                            c->d_tile_mats, c->qs_mask, c->d_air, c->d_air_list, c->d_air_count,
+                            c->ids, c->d_relax_tmp, state};
+            if (build_list) {
+                hip_check(c->K->air_mask(aa, s), "quasi-steady tile mask");
+                hip_check(c->K->compact(c->d_air, int(T), c->d_air_list, c->d_air_count, s),
+                          "quasi-steady tile list");
+            }
+            BatchStageArgs ra = a;
+            ra.list = c->d_air_list;
+            ra.count = c->d_air_count;
+            ra.queue = nullptr;
+            ra.strided = air_est;
+            ra.n_list = air_est + overflow_wgs(air_est);
+            ra.src = state;
+            ra.out = c->d_relax_tmp;
+            ra.U = state;
+            ra.V1 = nullptr;
+            ra.win = nullptr;
+            ra.wout = nullptr;
+            ra.coef = c->d_relax32;
+            ra.dose_hi = ra.dose_lo = nullptr;
+            ra.tile_max = nullptr;
+            ra.tile_sums = nullptr;
+            ra.tile_face = nullptr;
+            ra.judge = nullptr;
+            ra.bnd_stage = bnd_stage;
+            for (int it = 0; it < c->qs_iters; ++it) {
+                Timed t(c->prof, s, K_RELAX);
+                hip_check(c->K->stage(STAGE_RELAX, false, ra, s), "relaxation kernel");
+                hip_check(c->K->air_commit(aa, std::min(air_est, 16384), s), "relaxation commit");
+            }
         };
         int graph_est = 8;  // the estimate a device-stepped graph is captured with
         int graph_set[2] = {0, 0};  // the estimate each graph's stage grids hold
@@ -1277,6 +1385,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             const int all_tiles = round8(long(T));
             int est = all_tiles;  // the first step evaluates every tile
             int prev_count = int(T);
+            if (qs)
+                between_stages = [&](int done) {
+                    relax_air(cur, done == 0 ? c->W1 : c->W2, done + 1, done == 0);
+                };
             cfl_launch(cur);
             for (int64_t k = 0;; ++k) {
                 if (k >= R) {
@@ -1365,6 +1477,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                                             c->js_hi, c->js_lo, c->d_jdiff, c->d_jfull};
                         hip_check(c->K->compare(ca, int(c->max_box_cells), s), "judge compare");
                     }
+                    if (qs)
+                        relax_air(cur, c->U, 2, false);
                     advance_dev(jstep ? 1 : 0);
                     if (jstep)
                         slices(SLICE_RESTORE, c->d_sel + 2 * nb);
@@ -1725,6 +1839,22 @@ int m1_batch_graph_info(m1_batch* c, int64_t* graph_steps, const char** capture_
 int m1_batch_set_device_steps(m1_batch* c, int32_t enable)
 {
     return guarded(c, [&] { c->device_steps = enable != 0; });
+}
+
+int m1_batch_set_quasi_steady(m1_batch* c, uint32_t material_mask, int32_t iterations,
+                              double pseudo_cfl)
+{
+    return guarded(c, [&] {
+        if (material_mask >> MAXMAT)
+            throw ValueError{"material_mask names materials beyond the 8 supported"};
+        if (iterations < 0 || iterations > 4096)
+            throw ValueError{"iterations must lie in [0, 4096]"};
+        if (!(pseudo_cfl > 0.0 && pseudo_cfl <= 1.0))
+            throw ValueError{"pseudo_cfl must lie in (0, 1]"};
+        c->qs_mask = material_mask;
+        c->qs_iters = iterations;
+        c->qs_cfl = pseudo_cfl;
+    });
 }
 
 int m1_batch_physics_probe(m1_batch* c, int32_t material, int32_t kind, int32_t scattering,

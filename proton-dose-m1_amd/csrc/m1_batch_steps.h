@@ -65,6 +65,7 @@ struct DevStepArgs {
     int cfl_box_only;        // no outside-the-box materials in the CFL bound
     int use_model;
     int infer_every;
+    uint32_t qs_mask;        // quasi-steady materials: held through a step, no CFL bound
     double max_step, cfl, emin, tile_threshold;
     double dx, dy, dz;
     float judge_tol;

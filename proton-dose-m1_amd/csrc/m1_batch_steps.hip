@@ -49,8 +49,11 @@ __device__ inline int inflow_on(const BeamletConst& c, double max_step, double t
 __device__ inline void cfl_scalars(const DevStepArgs& a, int b, double energy, int tid)
 {
     StepCoef& c = a.cfl_coef[b];
+    // a quasi-steady material sets no CFL bound (its speeds count as 0)
     if (tid < a.n_mat)
-        c.S0[tid] = dev::stopping_power(a.mats[tid], energy, &a.ctr->err, tid, b);
+        c.S0[tid] = (a.qs_mask >> tid & 1u)
+                        ? HUGE_VAL
+                        : dev::stopping_power(a.mats[tid], energy, &a.ctr->err, tid, b);
     if (tid == 0) {
         const BeamletConst& q = a.bc[b];
         c.ed[0] = dev::energy_density(energy, q.energy, q.energy_spread);
@@ -265,6 +268,7 @@ __global__ void __launch_bounds__(SCALAR_THREADS) steps_scalars_kernel(DevStepAr
         sf.n1p[m] = sf.g1p[m];
         sf.n1j[m] = sf.g1j[m];
         sf.dose_scale[m] = float(alpha * MEV_TO_JOULE / sc.rho[m]);
+        sf.dEm[m] = (a.qs_mask >> m & 1u) ? 0.f : float(dE);
     }
     if (tid == MAXMAT * 4) {
         const double e13 = E - dE / 3.0, e23 = E - 2.0 * dE / 3.0, e1 = E - dE;

@@ -57,6 +57,10 @@ struct StepCoef32 {
     float n1p[MAXMAT], n1j[MAXMAT];     // C(tau(dE)) / S(E1): V_new -> U_new
     float S1[MAXMAT], S2[MAXMAT];
     float dose_scale[MAXMAT];           // alpha * MEV / rho
+    // step that multiplies the flux divergence: dE, or 0 for a quasi-steady
+    // material (held through the step and relaxed after it); in the relaxation
+    // coefficients, the pseudo-time CFL number of the quasi-steady materials
+    float dEm[MAXMAT];
     double eig_eps;
 };
 
@@ -70,7 +74,8 @@ enum StageKind : int {
     STAGE1 = 1,    // V2 = 3/4 S0 U + 1/4 (C1^-1 rhs(U1) dE + V1)
     STAGE2 = 2,    // U  = proj(C1 (1/3 S0 U + 2/3 (C2^-1 rhs(U2) dE + V2)) / S1), dose
     STAGE_RHS = 3, // out = rhs(U)  (spatial_operator)
-    STAGE_FE = 4   // out = proj((rhs(U) dE + S0 U) / S1), edep = dE U0 S0
+    STAGE_FE = 4,  // out = proj((rhs(U) dE + S0 U) / S1), edep = dE U0 S0
+    STAGE_RELAX = 5  // out = proj(U + dtau rhs(U)) on quasi-steady cells (batch)
 };
 
 struct StageArgs {
@@ -410,6 +415,25 @@ struct BatchLabelArgs {
     int cap;
 };
 
+// Quasi-steady materials (m1_batch_set_quasi_steady): the tiles holding one
+// of them, and the commit of a relaxation sweep.
+struct BatchAirArgs {
+    const BoxDesc* box;
+    const BoxStep* step;
+    const int* tile_box;
+    int n_tiles;
+    const uint8_t* live;       // tiles evaluated this step
+    const uint8_t* tile_mats;  // per tile: bit m set when material m occurs in it
+    uint32_t qs_mask;
+    uint8_t* air;              // out: live tiles of active boxes holding a quasi-steady material
+    // commit: the listed tiles' quasi-steady cells take `from` into `to`
+    const int* list;
+    const int* count;
+    const uint8_t* ids;
+    const float* from;
+    float* to;
+};
+
 // Judge: per box, the largest |full - skip| dose increment and the largest
 // full increment of one step (ordered float bits).
 struct BatchCompareArgs {
@@ -487,6 +511,11 @@ struct BatchKernels {
     // launch form of a stage kernel (0 one tile per workgroup, 1 overflow,
     // 2 queue), -1 when func is not one of this table's stage kernels
     int (*stage_form)(const void* func);
+    // quasi-steady tiles: the flags of the tiles to relax, and the commit of a
+    // relaxation sweep over the listed tiles (grid: workgroups per list entry
+    // estimate, each taking entries grid-stride)
+    hipError_t (*air_mask)(const BatchAirArgs& a, hipStream_t s);
+    hipError_t (*air_commit)(const BatchAirArgs& a, int grid, hipStream_t s);
 };
 
 // Tile shapes (x, y, z cells; y * z = 256 threads, x * y <= 256) the batched

@@ -905,7 +905,8 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
         if (valid) {
             const long i = cidx(x, y, z);
             const int m = a.ids[i];
-            const float dE = sc.dE;
+            const float dE = sc.dE;      // the step's energy width (dose)
+            const float dEr = sc.dEm[m];  // the step on the divergence (0: held)
             float rhs[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -916,12 +917,30 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     a.out[i + c * N] = rhs[c];
+            } else if constexpr (KIND == STAGE_RELAX) {
+                // a pseudo-time step towards div F = 0 of a quasi-steady cell,
+                // dtau = cfl / (sum over the axes of its speed / cell size)
+                if (dEr != 0.f) {
+                    const double sp = cfl_cell(a.src, i, N, 1.0, a.g, wave, eeps, false, 0.0, 1.0);
+                    const float dtau = sp > 0.0 ? float(double(dEr) / sp) : 0.f;
+                    float u[4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        u[c] = fmaf(rhs[c], dtau, W[2][c]);
+                    if constexpr (D)
+                        project_f(u);
+                    else
+                        project_ff(u);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        a.out[i + c * N] = u[c];
+                }
             } else if constexpr (KIND == STAGE0) {
                 const float S0 = sc.S0[m];
                 float v[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    v[c] = fmaf(rhs[c], dE, S0 * W[2][c]);
+                    v[c] = fmaf(rhs[c], dEr, S0 * W[2][c]);
                     a.out[i + c * N] = v[c];
                 }
                 if constexpr (PRE)
@@ -931,7 +950,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 float u[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    u[c] = fmaf(rhs[c], dE, S0 * W[2][c]) * (1.f / sc.S1[m]);
+                    u[c] = fmaf(rhs[c], dEr, S0 * W[2][c]) * (1.f / sc.S1[m]);
                 if constexpr (D)
                     project_f(u);
                 else
@@ -952,7 +971,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     float r = rhs[c];
                     if constexpr (IFAC)
                         r *= (c == 0 ? sc.i1p[m] : sc.i1j[m]);
-                    v[c] = fmaf(0.75f, S0 * a.U[i + c * N], 0.25f * fmaf(r, dE, a.V1[i + c * N]));
+                    v[c] = fmaf(0.75f, S0 * a.U[i + c * N], 0.25f * fmaf(r, dEr, a.V1[i + c * N]));
                     a.out[i + c * N] = v[c];
                 }
                 if constexpr (PRE)
@@ -969,7 +988,7 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     if constexpr (IFAC)
                         r *= (c == 0 ? sc.i2p[m] : sc.i2j[m]);
                     const float v = fmaf(1.f / 3.f, S0 * un[c],
-                                         (2.f / 3.f) * fmaf(r, dE, a.src[i + c * N]));
+                                         (2.f / 3.f) * fmaf(r, dEr, a.src[i + c * N]));
                     u[c] = v * (c == 0 ? sc.g1p[m] : sc.g1j[m]);
                 }
                 if constexpr (D)

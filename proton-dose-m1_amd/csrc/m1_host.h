@@ -295,6 +295,7 @@ inline void fill_step32(size_t nm, double alpha, const StepCoef& d, StepCoef32& 
         f.n1p[m] = f.g1p[m];
         f.n1j[m] = f.g1j[m];
         f.dose_scale[m] = float(alpha * MEV_TO_JOULE / d.rho[m]);
+        f.dEm[m] = f.dE;
     }
 }
 
@@ -310,14 +311,14 @@ inline double cfl_energy_step(const Num& n, double mx)
 // ------------------------------------------------------------ profiling
 enum KernelId {
     K_STAGE0, K_STAGE1, K_STAGE2, K_RHS, K_FE, K_CFL, K_BOUNDARY, K_SCALE, K_TILE_MASK,
-    K_TILE_ZERO, K_DOSE, K_MODEL, K_STEP_SCALARS, K_STEP_ADVANCE, K_COUNT
+    K_TILE_ZERO, K_DOSE, K_MODEL, K_STEP_SCALARS, K_STEP_ADVANCE, K_RELAX, K_COUNT
 };
 inline const char* kernel_name(int k)
 {
     static const char* const names[K_COUNT] = {
         "stage0", "stage1", "stage2", "spatial_operator", "forward_euler", "cfl", "boundary",
         "source_half_step", "tile_mask", "tile_zero", "dose", "tile_skip_network",
-        "step_scalars", "step_advance"};
+        "step_scalars", "step_advance", "quasi_steady_relax"};
     return k >= 0 && k < K_COUNT ? names[k] : "";
 }
 
