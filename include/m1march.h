@@ -400,6 +400,10 @@ int m1_batch_graph_info(m1_batch *batch, int64_t *graph_steps, const char **capt
  * (every judge_every-th step of the batch) rather than per beamlet, and not
  * on their last step.  Recording marches keep host-side stepping. */
 int m1_batch_set_device_steps(m1_batch *batch, int32_t enable);
+/* Progress of device-stepped marches (no reference counterpart): with
+ * every > 0 a line "m1 batch: energy step k, n beamlets marching" goes to
+ * stderr every `every` energy steps (0, the default: none). */
+int m1_batch_set_progress(m1_batch *batch, int32_t every);
 /* Quasi-steady materials (no reference counterpart; for phantoms holding
  * air, whose stopping power is 1/934 of water's so that an explicit energy
  * step over air is 934 times shorter).  Materials whose bit is set in

@@ -246,6 +246,7 @@ struct m1_batch {
     int* d_air_count = nullptr;
     float* d_relax_tmp = nullptr;     // 4 x cells: a relaxation sweep's result
     long tiles_with_mat[MAXMAT] = {};  // tiles of the boxes holding material m
+    int progress_every = 0;  // m1_batch_set_progress: a status line every N device steps
     // device-side energy stepping (m1_batch_set_device_steps)
     bool device_steps = false;
     DevMaterial* d_mats = nullptr;  // material tables on the device
@@ -1404,6 +1405,9 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     }
                     if (st_active == 0)
                         break;
+                    if (c->progress_every > 0 && (k - R) % c->progress_every == 0)
+                        std::fprintf(stderr, "m1 batch: energy step %lld, %d beamlets marching\n",
+                                     static_cast<long long>(k - R), st_active);
                     const long mx = std::max(st_count, prev_count);
                     prev_count = st_count;
                     est = std::min(all_tiles, round8(mx + mx / 4 + 64));
@@ -1839,6 +1843,15 @@ int m1_batch_graph_info(m1_batch* c, int64_t* graph_steps, const char** capture_
 int m1_batch_set_device_steps(m1_batch* c, int32_t enable)
 {
     return guarded(c, [&] { c->device_steps = enable != 0; });
+}
+
+int m1_batch_set_progress(m1_batch* c, int32_t every)
+{
+    return guarded(c, [&] {
+        if (every < 0)
+            throw ValueError{"every must be >= 0"};
+        c->progress_every = every;
+    });
 }
 
 int m1_batch_set_quasi_steady(m1_batch* c, uint32_t material_mask, int32_t iterations,

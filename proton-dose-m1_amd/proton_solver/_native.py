@@ -136,6 +136,7 @@ SIGNATURES = [
     ("m1_batch_set_cfl_tail_floor", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_set_device_steps", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_set_quasi_steady", C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_double]),
+    ("m1_batch_set_progress", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_physics_probe", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          _dp, C.c_double, C.c_int32, _dp]),
     ("m1_batch_set_skip_guard", C.c_int, [C.c_void_p, C.c_int32]),
@@ -645,6 +646,10 @@ class Batch:
         """Energies, CFL steps, step scalars and judge decisions on the device:
         steps are enqueued without a host round trip between them."""
         self._check(self.lib.m1_batch_set_device_steps(self.h, int(bool(enable))))
+
+    def set_progress(self, every: int):
+        """A status line on stderr every `every` device-stepped energy steps (0: none)."""
+        self._check(self.lib.m1_batch_set_progress(self.h, int(every)))
 
     def set_quasi_steady(self, material_mask: int, iterations: int, pseudo_cfl: float = 0.5):
         """Hold the materials of material_mask through each energy step and
