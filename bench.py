@@ -726,10 +726,11 @@ def run_combined(args):
 
 def air_validation(args):
     """The quasi-steady air treatment against the plain FP64 march through a
-    10 mm air pocket (48 x 48 x 64 cells at 1 mm, 60 MeV), where the plain
-    march takes air-bound energy steps: the check of the method the combined
-    workload relies on (its own plain FP64 march would take hours per
-    beamlet)."""
+    60 mm air pocket (semi-axes 12, 12, 30 mm at 15-75 mm depth in 48 x 48 x
+    160 cells of water at 1 mm, 100 MeV), where the plain march takes
+    air-bound energy steps: the check of the method the combined workload
+    relies on, on a pocket as long along the beam as its longest (its own
+    plain FP64 march would take hours per beamlet)."""
     import dataclasses
 
     from proton_solver import Material, _native, metrics
@@ -737,15 +738,16 @@ def air_validation(args):
     from proton_solver.batch import (Beamlet, Ellipsoid, batch_workload_config, ellipsoid_phantom,
                                      full_problem, prepare_batch)
 
-    ph = ellipsoid_phantom((48, 48, 64), (0.001,) * 3,
-                           [Ellipsoid((0.0, 0.0, 0.015), (0.008, 0.008, 0.005), Material.AIR)])
-    beamlets = [Beamlet(energy=60.0, sigma=(0.003, 0.003), center=(0.0, 0.0))]
+    ph = ellipsoid_phantom((48, 48, 160), (0.001,) * 3,
+                           [Ellipsoid((0.0, 0.0, 0.045), (0.012, 0.012, 0.030), Material.AIR)])
+    beamlets = [Beamlet(energy=100.0, sigma=(0.003, 0.003), center=(0.0, 0.0))]
     cfg = batch_workload_config()
     cfg = dataclasses.replace(cfg, numerics=dataclasses.replace(
         cfg.numerics, max_energy_step=args.air_max_step))
-    handle, _ = prepare_batch(ph, beamlets, cfg, boxes=[(0, 0, 48, 48, 64)], tile_shape=2)
+    handle, _ = prepare_batch(ph, beamlets, cfg, boxes=[(0, 0, 48, 48, 160)], tile_shape=2)
     set_mode(handle, "device")
     handle.set_quasi_steady(1 << ph.materials.index(Material.AIR), args.qs_iters, args.qs_cfl)
+    handle.set_quasi_steady_tolerance(args.qs_tol)
     t0 = time.perf_counter()
     _, steps = handle.march(_native.numerics_struct(cfg), want_dose=False)
     qs_s = time.perf_counter() - t0
@@ -754,12 +756,15 @@ def air_validation(args):
     ctx = _native.Context(0)
     ctx.set_problem(*full_problem(ph, beamlets[0]))
     ctx.set_state(None)
+    print(f"air validation: quasi-steady march {int(steps[0])} steps in {qs_s:.2f} s; "
+          "plain FP64 march", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     E64, _ = ctx.march(_native.numerics_struct(cfg, _native.FP64))
     fp64_s = time.perf_counter() - t0
     ref = ctx.get_dose()
     return {
-        "problem": "60 MeV beam through a 10 mm air pocket at 10-20 mm depth, 48x48x64 @1 mm water",
+        "problem": "100 MeV beam through a 60 mm air pocket at 15-75 mm depth, 48x48x160 @1 mm "
+                   "water",
         "reference": "plain FP64 march (air-bound steps; bitwise the CPU oracle)",
         "fp64_energy_steps": len(E64) - 1, "fp64_seconds": round(fp64_s, 2),
         "quasi_steady_energy_steps": int(steps[0]), "quasi_steady_seconds": round(qs_s, 2),
@@ -779,7 +784,8 @@ def run_combined_quasi_steady(args):
     FP64 march through an air pocket (air_validation), and air-free beamlets
     of the batch against the FP64 moving-domain march at the same maximum
     step (the plain FP64 march of this phantom is air-bound: ~1e-3 MeV steps,
-    hours per beamlet)."""
+    hours per beamlet).  Not bound by --time-budget: two marches of the whole
+    batch and the checks take several minutes."""
     import dataclasses
 
     from proton_solver import Material, _native, metrics
@@ -801,7 +807,8 @@ def run_combined_quasi_steady(args):
     handle, boxes = prepare_batch(phantom, beamlets, cfg, boxes, tile_shape=2)
     set_mode(handle, "device")
     handle.set_quasi_steady(1 << air, args.qs_iters, args.qs_cfl)
-    handle.set_progress(500)
+    handle.set_quasi_steady_tolerance(args.qs_tol)
+    handle.set_progress(250)
     num = _native.numerics_struct(cfg)
     # profiled march (untimed): kernel times and the roofline
     handle.set_profiling(True)
@@ -834,17 +841,21 @@ def run_combined_quasi_steady(args):
 
     # FP64 moving-domain samples of air-free beamlets, at the batch's maximum step
     rows = []
-    if args.fp64_sample > 0 and air_free:
-        order = sorted(air_free, key=lambda b: beamlets[b].energy)
+    # only beamlets that stop inside the phantom: past the far face the moving
+    # domain of an exiting beam widens by ~50 cells and reaches air (a 227 MeV
+    # beamlet's FP64 march drops to ~4e-4 MeV steps at 85 MeV there)
+    stopping = [b for b in air_free if boxes[b][4] < phantom.shape[2]]
+    if args.fp64_sample > 0 and stopping:
+        order = sorted(stopping, key=lambda b: beamlets[b].energy)
         picks = sorted({order[int(round(q * (len(order) - 1)))]
                         for q in np.linspace(0.0, 1.0, args.fp64_sample)})
         cfg64 = dataclasses.replace(cfg, numerics=dataclasses.replace(cfg.numerics,
                                                                       vacuum_aware_cfl=True))
         num64 = _native.numerics_struct(cfg64, _native.FP64)
         ctx = _native.Context(0)
-        for b in picks:
-            if spent() > args.time_budget - 60:
-                break
+        for b in picks:  # about a second each (air-free, stopping inside the phantom)
+            print(f"fp64 moving-domain march of beamlet {b} ({beamlets[b].energy:.1f} MeV)",
+                  file=sys.stderr, flush=True)
             ctx.set_problem(*full_problem(phantom, beamlets[b]))
             ctx.set_state(None)
             t1 = time.perf_counter()
@@ -895,8 +906,8 @@ def run_combined_quasi_steady(args):
             "n_beamlets": B,
             "boxes_holding_air": air_boxes,
             "air_free_beamlets": len(air_free),
-            "quasi_steady": {"material": "AIR", "sweeps_per_stage": args.qs_iters,
-                             "pseudo_cfl": args.qs_cfl},
+            "quasi_steady": {"material": "AIR", "max_sweeps_per_stage": args.qs_iters,
+                             "tolerance": args.qs_tol, "pseudo_cfl": args.qs_cfl},
             "tile_shape": list(handle.tile_shape),
             "energy_steps_per_beamlet": {"min": int(steps.min()), "mean": float(steps.mean()),
                                          "max": int(steps.max())},
@@ -1067,8 +1078,11 @@ def main():
                     help="combined workload: march only the beamlets with no air near their "
                          "entry column (vacuum-aware CFL, network, graph steps) instead of all "
                          "of them with the air quasi-steady")
-    ap.add_argument("--qs-iters", type=int, default=64,
-                    help="quasi-steady air: relaxation sweeps after each stage")
+    ap.add_argument("--qs-iters", type=int, default=256,
+                    help="quasi-steady air: at most this many relaxation sweeps after each stage")
+    ap.add_argument("--qs-tol", type=float, default=1e-6,
+                    help="quasi-steady air: a beamlet stops sweeping once no air cell changes "
+                         "by more than this x its peak inflow fluence (0: every sweep)")
     ap.add_argument("--qs-cfl", type=float, default=0.3,
                     help="quasi-steady air: pseudo-time CFL number of a sweep")
     ap.add_argument("--air-max-step", type=float, default=0.1,

@@ -407,17 +407,24 @@ int m1_batch_set_progress(m1_batch *batch, int32_t every);
 /* Quasi-steady materials (no reference counterpart; for phantoms holding
  * air, whose stopping power is 1/934 of water's so that an explicit energy
  * step over air is 934 times shorter).  Materials whose bit is set in
- * material_mask hold their state through each energy step (their flux
- * divergence is not applied and they set no CFL bound); after the step
- * `iterations` Jacobi sweeps of a pseudo-time step towards div F = 0
- * (forward Euler, pseudo-time CFL number pseudo_cfl with each cell's own M1
- * speeds) relax their cells with the other cells fixed: the beam crosses
- * them at no energy cost, the limit of the reference's march as their
- * stopping power goes to 0.  Their dose is deposited from the held state.
- * Needs device-side stepping; not combined with the tile-skip network.
- * iterations 0 turns it off (the default). */
+ * material_mask hold their state through each stage of an energy step
+ * (their flux divergence is not applied and they set no CFL bound); after
+ * each stage `iterations` Jacobi sweeps of a pseudo-time step towards
+ * div F = 0 (forward Euler, pseudo-time CFL number pseudo_cfl with each
+ * cell's own M1 speeds) relax their cells of the next stage's input with the
+ * other cells fixed: the beam crosses them at no energy cost, the limit of
+ * the reference's march as their stopping power goes to 0.  Their dose is
+ * deposited from the held and relaxed states.  Needs device-side stepping;
+ * not combined with the tile-skip network.  iterations 0 turns it off (the
+ * default). */
 int m1_batch_set_quasi_steady(m1_batch *batch, uint32_t material_mask, int32_t iterations,
                               double pseudo_cfl);
+/* Convergence of the quasi-steady sweeps (no reference counterpart): with
+ * tolerance > 0 a beamlet stops sweeping after a stage once a sweep changed
+ * none of its quasi-steady cells by more than tolerance times its peak
+ * inflow fluence (at most `iterations` sweeps); 0, the default, runs every
+ * sweep.  Tolerance in [0, 1). */
+int m1_batch_set_quasi_steady_tolerance(m1_batch *batch, double tolerance);
 /* Device evaluation of the step physics of the bound problem's material
  * `material` at n energies (test access; no reference counterpart): kind 0
  * S(E) (stopping_power.py:141-228), 1 the transport coefficient of

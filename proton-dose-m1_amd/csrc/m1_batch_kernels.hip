@@ -53,6 +53,10 @@ template <int KIND, bool IFAC, bool D>
 __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
 {
     const int b = a.tile_box[t];
+    if constexpr (KIND == STAGE_RELAX) {
+        if (a.relax_done && a.relax_done[b])
+            return;  // the box's sweeps have converged (uniform over the workgroup)
+    }
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
     const long off4 = 4 * d.cell_off;
@@ -74,7 +78,9 @@ __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
                a.ids + d.cell_off,
                a.bnd + 12 * d.col_off + 4 * m2 * a.bnd_stage,
                a.coef + b,
-               a.tile_max ? a.tile_max + t : nullptr,
+               KIND == STAGE_RELAX
+                   ? (a.relax_change ? reinterpret_cast<float*>(a.relax_change + b) : nullptr)
+                   : (a.tile_max ? a.tile_max + t : nullptr),
                a.tile_sums ? a.tile_sums + 2 * long(t) : nullptr,
                a.tile_face ? a.tile_face + FACE_SLOTS * long(t) : nullptr,
                a.wave_mode,
@@ -1081,13 +1087,25 @@ __global__ void air_mask_kernel(BatchAirArgs a)
     a.air[t] = a.live[t] && a.step[a.tile_box[t]].active && (a.tile_mats[t] & a.qs_mask) ? 1 : 0;
 }
 
-// The listed tiles' quasi-steady cells take the relaxed state (from -> to).
+// The listed tiles' quasi-steady cells take the relaxed state (from -> to);
+// with convergence flags, boxes done before this sweep are left as they are,
+// and each box's flag for the next sweep is set here (every workgroup of a
+// box writes the same values).
 __global__ void __launch_bounds__(NT) air_commit_kernel(BatchAirArgs a)
 {
     const int n = *a.count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const int t = a.list[k];
         const int b = a.tile_box[t];
+        if (a.done) {
+            const bool was = a.done[b] != 0;
+            if (threadIdx.x == 0) {
+                a.next_done[b] = was || __uint_as_float(a.change[b]) < a.tol ? 1 : 0;
+                a.next_change[b] = 0u;
+            }
+            if (was)
+                continue;
+        }
         const BoxDesc d = a.box[b];
         const TileCoord tc = tile_coord(d, b, t);
         const long N = long(d.nx) * d.ny * d.nz;
@@ -1107,11 +1125,29 @@ __global__ void __launch_bounds__(NT) air_commit_kernel(BatchAirArgs a)
     }
 }
 
+__global__ void air_reset_kernel(BatchAirArgs a)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.n_boxes)
+        return;
+    a.next_done[b] = 0;
+    a.next_change[b] = 0u;
+}
+
 hipError_t launch_air_mask(const BatchAirArgs& a, hipStream_t s)
 {
     if (a.n_tiles <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(air_mask_kernel, dim3(unsigned((a.n_tiles + 255) / 256)), dim3(256), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
+hipError_t launch_air_reset(const BatchAirArgs& a, hipStream_t s)
+{
+    if (a.n_boxes <= 0 || !a.next_done)
+        return hipSuccess;
+    hipLaunchKernelGGL(air_reset_kernel, dim3(unsigned((a.n_boxes + 255) / 256)), dim3(256), 0, s,
                        a);
     return hipGetLastError();
 }
@@ -1142,7 +1178,7 @@ const BatchKernels& M1_BATCH_TABLE()
                                 &launch_scatter_dose, &launch_box_dose, &launch_features,
                                 &launch_labels, &launch_compare, &launch_slices,
                                 &launch_clear,  &stage_blocks_per_cu, &stage_form,
-                                &launch_air_mask, &launch_air_commit};
+                                &launch_air_mask, &launch_air_commit, &launch_air_reset};
     return k;
 }
 }  // namespace m1

@@ -245,6 +245,9 @@ struct m1_batch {
     int* d_air_list = nullptr;
     int* d_air_count = nullptr;
     float* d_relax_tmp = nullptr;     // 4 x cells: a relaxation sweep's result
+    double qs_tol = 0.0;              // per-box convergence of the sweeps (0: fixed count)
+    uint8_t* d_relax_done = nullptr;  // 2 x boxes: converged before the sweep / after it
+    unsigned* d_relax_change = nullptr;  // 2 x boxes: a sweep's largest change (float bits)
     long tiles_with_mat[MAXMAT] = {};  // tiles of the boxes holding material m
     int progress_every = 0;  // m1_batch_set_progress: a status line every N device steps
     // device-side energy stepping (m1_batch_set_device_steps)
@@ -318,11 +321,13 @@ void m1_batch::free_problem()
             (void)hipFree(p);
     for (void* p : {(void*)d_mats, (void*)d_tables, (void*)d_bc, (void*)d_bst, (void*)d_ctr,
                     (void*)d_relax32, (void*)d_air, (void*)d_air_list, (void*)d_air_count,
-                    (void*)d_relax_tmp})
+                    (void*)d_relax_tmp, (void*)d_relax_done, (void*)d_relax_change})
         if (p)
             (void)hipFree(p);
     d_relax32 = nullptr;
     d_relax_tmp = nullptr;
+    d_relax_done = nullptr;
+    d_relax_change = nullptr;
     d_air = nullptr;
     d_air_list = d_air_count = nullptr;
     d_mats = nullptr;
@@ -904,6 +909,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     dev_alloc(&c->d_air_list, T, "hipMalloc quasi-steady tile list");
                     dev_alloc(&c->d_air_count, LIST_HEADER_ALLOC, "hipMalloc tile count");
                     dev_alloc(&c->d_relax_tmp, 4 * N, "hipMalloc relaxation state");
+                    dev_alloc(&c->d_relax_done, 2 * nb, "hipMalloc relaxation flags");
+                    dev_alloc(&c->d_relax_change, 2 * nb, "hipMalloc relaxation changes");
                 }
                 std::vector<StepCoef32> rc(nb);
                 for (size_t b = 0; b < nb; ++b) {
@@ -1173,10 +1180,17 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             if (c->qs_mask >> m & 1u)
                 air_bound += c->tiles_with_mat[m];
         const int air_est = int(std::max(8L, std::min(long(T), (air_bound + 7) / 8 * 8)));
+        // With a tolerance, a box stops sweeping once a sweep changes none of
+        // its quasi-steady cells by more than it (the float state is scaled
+        // to the beamlet's peak inflow fluence, so the bound is relative to
+        // each beamlet): the flags and changes alternate between two halves.
+        const bool qs_conv = c->qs_tol > 0.0;
         auto relax_air = [&](int k, float* state, int bnd_stage, bool build_list) {
             BatchAirArgs aa{c->d_box, c->d_step, c->d_tile_box, int(T), c->d_flags[k],
                             c->d_tile_mats, c->qs_mask, c->d_air, c->d_air_list, c->d_air_count,
                             c->ids, c->d_relax_tmp, state};
+            aa.n_boxes = int(B);
+            aa.tol = float(c->qs_tol);
             if (build_list) {
                 hip_check(c->K->air_mask(aa, s), "quasi-steady tile mask");
                 hip_check(c->K->compact(c->d_air, int(T), c->d_air_list, c->d_air_count, s),
@@ -1201,7 +1215,19 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             ra.tile_face = nullptr;
             ra.judge = nullptr;
             ra.bnd_stage = bnd_stage;
+            if (qs_conv) {
+                aa.next_done = c->d_relax_done;
+                aa.next_change = c->d_relax_change;
+                hip_check(c->K->air_reset(aa, s), "relaxation flag reset");
+            }
             for (int it = 0; it < c->qs_iters; ++it) {
+                if (qs_conv) {
+                    const int p = it & 1;
+                    aa.done = ra.relax_done = c->d_relax_done + p * B;
+                    aa.change = ra.relax_change = c->d_relax_change + p * B;
+                    aa.next_done = c->d_relax_done + (1 - p) * B;
+                    aa.next_change = c->d_relax_change + (1 - p) * B;
+                }
                 Timed t(c->prof, s, K_RELAX);
                 hip_check(c->K->stage(STAGE_RELAX, false, ra, s), "relaxation kernel");
                 hip_check(c->K->air_commit(aa, std::min(air_est, 16384), s), "relaxation commit");
@@ -1867,6 +1893,15 @@ int m1_batch_set_quasi_steady(m1_batch* c, uint32_t material_mask, int32_t itera
         c->qs_mask = material_mask;
         c->qs_iters = iterations;
         c->qs_cfl = pseudo_cfl;
+    });
+}
+
+int m1_batch_set_quasi_steady_tolerance(m1_batch* c, double tolerance)
+{
+    return guarded(c, [&] {
+        if (!(tolerance >= 0.0 && tolerance < 1.0))
+            throw ValueError{"tolerance must lie in [0, 1)"};
+        c->qs_tol = tolerance;
     });
 }
 

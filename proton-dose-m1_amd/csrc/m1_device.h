@@ -301,6 +301,11 @@ struct BatchStageArgs {
     // one this stage forms for the next (wout); nullable.
     const float* win;
     float* wout;
+    // Relaxation sweeps (STAGE_RELAX; nullable): boxes flagged in
+    // relax_done skip the sweep, the others fold their quasi-steady cells'
+    // largest change into relax_change[box] (ordered float bits).
+    const uint8_t* relax_done;
+    unsigned* relax_change;
 };
 
 // Per list: its length followed by one set of XCD queue counters per stage
@@ -432,6 +437,16 @@ struct BatchAirArgs {
     const uint8_t* ids;
     const float* from;
     float* to;
+    // Convergence of the sweeps per box (nullable: a fixed sweep count).  The
+    // commit of a sweep skips boxes flagged in done, and sets next_done of a
+    // box when it was done or the sweep's largest change (change, ordered
+    // float bits) fell below tol; it clears next_change for the next sweep.
+    int n_boxes;
+    const uint8_t* done;
+    uint8_t* next_done;
+    const unsigned* change;
+    unsigned* next_change;
+    float tol;
 };
 
 // Judge: per box, the largest |full - skip| dose increment and the largest
@@ -516,6 +531,8 @@ struct BatchKernels {
     // estimate, each taking entries grid-stride)
     hipError_t (*air_mask)(const BatchAirArgs& a, hipStream_t s);
     hipError_t (*air_commit)(const BatchAirArgs& a, int grid, hipStream_t s);
+    // before a series of sweeps: done[b] = 0, change[b] = 0 for every box
+    hipError_t (*air_reset)(const BatchAirArgs& a, hipStream_t s);
 };
 
 // Tile shapes (x, y, z cells; y * z = 256 threads, x * y <= 256) the batched

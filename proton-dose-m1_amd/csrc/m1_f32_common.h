@@ -29,7 +29,7 @@ struct TileView {
     const uint8_t* ids;
     const float* bnd;
     const StepCoef32* coef;
-    float* tile_max;   // this tile's slot (nullable)
+    float* tile_max;   // this tile's slot (nullable); relaxation: the box's change slot
     float* tile_sums;  // this tile's sum phi, sum |J| (nullable)
     float* tile_face;  // this tile's FACE_SLOTS face slabs (nullable; needs tile_max)
     int wave_mode;
@@ -932,8 +932,10 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                     else
                         project_ff(u);
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
+                    for (int c = 0; c < 4; ++c) {
                         a.out[i + c * N] = u[c];
+                        phimax = fmaxf(phimax, fabsf(u[c] - W[2][c]));  // the sweep's change
+                    }
                 }
             } else if constexpr (KIND == STAGE0) {
                 const float S0 = sc.S0[m];
@@ -1030,6 +1032,16 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 W[k][c] = W[k + 1][c];
+    }
+    if constexpr (KIND == STAGE_RELAX) {
+        // largest change of a quasi-steady cell this sweep, folded into the
+        // box's slot as ordered bits (non-negative floats order like them)
+        if (a.tile_max) {
+            for (int off = 32; off > 0; off >>= 1)
+                phimax = fmaxf(phimax, __shfl_xor(phimax, off, 64));
+            if ((tid & 63) == 0)
+                atomicMax(reinterpret_cast<unsigned*>(a.tile_max), __float_as_uint(phimax));
+        }
     }
     if constexpr (KIND == STAGE2) {
         if (a.tile_max) {

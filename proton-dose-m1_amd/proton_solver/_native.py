@@ -136,6 +136,7 @@ SIGNATURES = [
     ("m1_batch_set_cfl_tail_floor", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_set_device_steps", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_set_quasi_steady", C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_double]),
+    ("m1_batch_set_quasi_steady_tolerance", C.c_int, [C.c_void_p, C.c_double]),
     ("m1_batch_set_progress", C.c_int, [C.c_void_p, C.c_int32]),
     ("m1_batch_physics_probe", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          _dp, C.c_double, C.c_int32, _dp]),
@@ -656,6 +657,12 @@ class Batch:
         relax them towards div F = 0 after it (`iterations` sweeps; 0: off)."""
         self._check(self.lib.m1_batch_set_quasi_steady(self.h, int(material_mask),
                                                         int(iterations), float(pseudo_cfl)))
+
+    def set_quasi_steady_tolerance(self, tolerance: float):
+        """Stop a beamlet's sweeps after a stage once no quasi-steady cell
+        changes by more than tolerance x its peak inflow fluence (0: every
+        sweep)."""
+        self._check(self.lib.m1_batch_set_quasi_steady_tolerance(self.h, float(tolerance)))
 
     def physics_probe(self, material: int, kind: int, energies, dE: float = 0.0,
                       scattering: int = 2, interpolation: int = 0):

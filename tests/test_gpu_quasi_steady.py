@@ -10,8 +10,9 @@ through each stage and relaxes them towards div F = 0 after it.  Checked
 against that plain FP64 march (bitwise the CPU oracle): maximum dose within
 0.5 %, gamma(1 %/1 mm) >= 99.5 %, with a 0.1 MeV maximum step (the plain
 march's own energy grid is ~1e-3 MeV here; at 0.25 MeV the water alone
-differs from a converged energy grid by ~0.5 % in maximum dose).  Also the
-argument checks.
+differs from a converged energy grid by ~0.5 % in maximum dose).  The same
+with the sweeps stopping per beamlet at a tolerance
+(m1_batch_set_quasi_steady_tolerance).  Also the argument checks.
 """
 
 import dataclasses
@@ -39,26 +40,40 @@ def _problem(max_step=0.1):
     return ph, beamlets, cfg
 
 
-def test_quasi_steady_air_against_the_plain_fp64_march():
+@pytest.fixture(scope="module")
+def plain_fp64():
     ph, beamlets, cfg = _problem()
-    air = ph.materials.index(Material.AIR)
-    handle, _ = prepare_batch(ph, beamlets, cfg, boxes=[(0, 0, 48, 48, 64)], tile_shape=2)
-    set_mode(handle, "device")
-    handle.set_quasi_steady(1 << air, 64, 0.3)
-    _, steps = handle.march(_native.numerics_struct(cfg), want_dose=False)
-    got = handle.beamlet_dose(0)
     ctx = _native.Context(0)
     ctx.set_problem(*full_problem(ph, beamlets[0]))
     ctx.set_state(None)
     say("plain FP64 march through the air pocket (air-bound steps)")
     E64, _ = ctx.march(_native.numerics_struct(cfg, _native.FP64))
-    ref = ctx.get_dose()
+    return len(E64) - 1, ctx.get_dose()
+
+
+def _quasi_steady(sweeps, tolerance):
+    ph, beamlets, cfg = _problem()
+    air = ph.materials.index(Material.AIR)
+    handle, _ = prepare_batch(ph, beamlets, cfg, boxes=[(0, 0, 48, 48, 64)], tile_shape=2)
+    set_mode(handle, "device")
+    handle.set_quasi_steady(1 << air, sweeps, 0.3)
+    handle.set_quasi_steady_tolerance(tolerance)
+    _, steps = handle.march(_native.numerics_struct(cfg), want_dose=False)
+    got = handle.beamlet_dose(0)
+    handle.close()
+    return ph, int(steps[0]), got
+
+
+@pytest.mark.parametrize("sweeps,tolerance", [(64, 0.0), (256, 1e-6)])
+def test_quasi_steady_air_against_the_plain_fp64_march(plain_fp64, sweeps, tolerance):
+    n64, ref = plain_fp64
+    ph, steps, got = _quasi_steady(sweeps, tolerance)
     err = metrics.maximum_dose_relative_difference(got, ref)
     diff = metrics.max_abs_difference_over_max(got, ref)
     gamma = metrics.gamma_pass_rate(got, ref, ph.spacing, 1.0, 1.0, 10.0, device=0)
-    say(f"quasi-steady air: {int(steps[0])} steps against {len(E64) - 1}; max-dose error "
-        f"{err:+.2e}, max |dD| / max {diff:.2e}, gamma {gamma:.3f} %")
-    assert len(E64) - 1 > 50 * int(steps[0])
+    say(f"quasi-steady air ({sweeps} sweeps, tolerance {tolerance:g}): {steps} steps against "
+        f"{n64}; max-dose error {err:+.2e}, max |dD| / max {diff:.2e}, gamma {gamma:.3f} %")
+    assert n64 > 50 * steps
     assert abs(err) <= 5e-3
     assert gamma >= 99.5
     # the dose before the pocket (z < 9 mm) is the water march's
@@ -73,6 +88,9 @@ def test_quasi_steady_argument_checks():
     for bad in ((1 << 8, 8, 0.5), (1 << air, -1, 0.5), (1 << air, 8, 0.0), (1 << air, 8, 1.5)):
         with pytest.raises(ValueError):
             handle.set_quasi_steady(*bad)
+    for bad in (-1e-6, 1.0, float("nan")):
+        with pytest.raises(ValueError):
+            handle.set_quasi_steady_tolerance(bad)
     handle.set_quasi_steady(1 << air, 8, 0.3)
     set_mode(handle, "per_tile")  # host-side stepping
     with pytest.raises(NotImplementedError, match="device-side stepping"):
