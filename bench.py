@@ -1078,9 +1078,9 @@ def main():
                     help="combined workload: march only the beamlets with no air near their "
                          "entry column (vacuum-aware CFL, network, graph steps) instead of all "
                          "of them with the air quasi-steady")
-    ap.add_argument("--qs-iters", type=int, default=256,
+    ap.add_argument("--qs-iters", type=int, default=128,
                     help="quasi-steady air: at most this many relaxation sweeps after each stage")
-    ap.add_argument("--qs-tol", type=float, default=1e-6,
+    ap.add_argument("--qs-tol", type=float, default=1e-5,
                     help="quasi-steady air: a beamlet stops sweeping once no air cell changes "
                          "by more than this x its peak inflow fluence (0: every sweep)")
     ap.add_argument("--qs-cfl", type=float, default=0.3,
