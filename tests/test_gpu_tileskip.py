@@ -113,10 +113,11 @@ def test_recorded_samples(samples, cfg):
 
 @pytest.fixture(scope="module")
 def trained(samples):
-    """A network trained on 90 % of the samples; the rest are held out."""
+    """A network trained on 80 % of the samples; the rest are held out (about
+    12,000 candidates, so that one false skip is below 1e-4 of them)."""
     x, y, near = samples
     order = np.random.default_rng(0).permutation(len(y))
-    n_test = len(y) // 10
+    n_test = len(y) // 5
     fit, test = order[n_test:], order[:n_test]
     model, loss = TS.train(x[fit], y[fit], epochs=20, batch=2048, lr=1e-3)
     say("training loss", loss)
@@ -129,7 +130,8 @@ def test_network_beats_the_fixed_guard_rule(trained):
     the network, at a decision probability with no more false skips than
     1e-4 of the candidates, leaves out at least 1.5 times as many."""
     model, (x, y, near) = trained
-    report = TS.against_guard_rule(model, x, y, near, decisions=(0.9, 0.99, 0.995, 0.999))
+    report = TS.against_guard_rule(model, x, y, near,
+                                   decisions=(0.9, 0.99, 0.995, 0.999, 0.9999, 0.99999))
     for r in report["rules"]:
         say(r)
     rule = next(r for r in report["rules"] if r["rule"] == "guard 4 cells")
