@@ -49,13 +49,53 @@ __device__ __forceinline__ TileCoord tile_coord(const BoxDesc& d, int b, int t)
     return {b, local / (d.ntz * d.nty), (local / d.ntz) % d.nty, local % d.ntz};
 }
 
+// The first relaxation sweep of a box after it is done: the tile's
+// quasi-steady cells keep their values (V1, or src on the first sweep, into
+// out).
+__device__ void relax_copy_tile(const BatchStageArgs& a, int b, int t)
+{
+    const BoxDesc d = a.box[b];
+    const TileCoord tc = tile_coord(d, b, t);
+    const long N = long(d.nx) * d.ny * d.nz;
+    const int z = tc.tz * TZ + threadIdx.x % TZ;
+    const int y = tc.ty * TY + threadIdx.x / TZ;
+    if (z >= d.nz || y >= d.ny)
+        return;
+    const long off4 = 4 * d.cell_off;
+    const float* from = (a.V1 ? a.V1 : a.src) + off4;
+    float* to = a.out + off4;
+    const StepCoef32* c = a.coef + b;
+    for (int x = tc.tx * TX; x < min(tc.tx * TX + TX, d.nx); ++x) {
+        const long i = (long(x) * d.ny + y) * d.nz + z;
+        if (c->dEm[a.ids[d.cell_off + i]] == 0.f)
+            continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            to[i + q * N] = from[i + q * N];
+    }
+}
+
 template <int KIND, bool IFAC, bool D>
 __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
 {
     const int b = a.tile_box[t];
     if constexpr (KIND == STAGE_RELAX) {
-        if (a.relax_done && a.relax_done[b])
-            return;  // the box's sweeps have converged (uniform over the workgroup)
+        if (a.relax_done_in) {
+            // the same decision in every workgroup of the box; a box that has
+            // just become done copies its cells into this sweep's output, so
+            // that both outputs hold them and later sweeps leave the box alone
+            const bool was = a.relax_done_in[b] != 0;
+            const bool done = was || __uint_as_float(a.relax_change_prev[b]) < a.relax_tol;
+            if (threadIdx.x == 0) {
+                a.relax_done_out[b] = done ? 1 : 0;
+                a.relax_change_clear[b] = 0u;
+            }
+            if (done) {
+                if (!was)
+                    relax_copy_tile(a, b, t);
+                return;
+            }
+        }
     }
     const BoxDesc d = a.box[b];
     const TileCoord tc = tile_coord(d, b, t);
@@ -79,7 +119,7 @@ __device__ __forceinline__ void run_tile(const BatchStageArgs& a, int t)
                a.bnd + 12 * d.col_off + 4 * m2 * a.bnd_stage,
                a.coef + b,
                KIND == STAGE_RELAX
-                   ? (a.relax_change ? reinterpret_cast<float*>(a.relax_change + b) : nullptr)
+                   ? (a.relax_done_in ? reinterpret_cast<float*>(a.relax_change_cur + b) : nullptr)
                    : (a.tile_max ? a.tile_max + t : nullptr),
                a.tile_sums ? a.tile_sums + 2 * long(t) : nullptr,
                a.tile_face ? a.tile_face + FACE_SLOTS * long(t) : nullptr,
@@ -821,6 +861,10 @@ __device__ __forceinline__ void zero_tile(const BatchZeroArgs& a, int t)
                 a.W1[off4 + i + c * N] = 0.f;
                 a.W2[off4 + i + c * N] = 0.f;
             }
+            if (a.R1) {
+                a.R1[off4 + i + c * N] = 0.f;
+                a.R2[off4 + i + c * N] = 0.f;
+            }
         }
     }
 }
@@ -1087,25 +1131,13 @@ __global__ void air_mask_kernel(BatchAirArgs a)
     a.air[t] = a.live[t] && a.step[a.tile_box[t]].active && (a.tile_mats[t] & a.qs_mask) ? 1 : 0;
 }
 
-// The listed tiles' quasi-steady cells take the relaxed state (from -> to);
-// with convergence flags, boxes done before this sweep are left as they are,
-// and each box's flag for the next sweep is set here (every workgroup of a
-// box writes the same values).
+// The listed tiles' quasi-steady cells take the relaxed state (from -> to).
 __global__ void __launch_bounds__(NT) air_commit_kernel(BatchAirArgs a)
 {
     const int n = *a.count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const int t = a.list[k];
         const int b = a.tile_box[t];
-        if (a.done) {
-            const bool was = a.done[b] != 0;
-            if (threadIdx.x == 0) {
-                a.next_done[b] = was || __uint_as_float(a.change[b]) < a.tol ? 1 : 0;
-                a.next_change[b] = 0u;
-            }
-            if (was)
-                continue;
-        }
         const BoxDesc d = a.box[b];
         const TileCoord tc = tile_coord(d, b, t);
         const long N = long(d.nx) * d.ny * d.nz;
@@ -1130,8 +1162,9 @@ __global__ void air_reset_kernel(BatchAirArgs a)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= a.n_boxes)
         return;
-    a.next_done[b] = 0;
-    a.next_change[b] = 0u;
+    a.done[b] = 0;
+    a.change[b] = 0u;
+    a.change[2 * a.n_boxes + b] = 0x7f800000u;  // +inf: the first sweep runs
 }
 
 hipError_t launch_air_mask(const BatchAirArgs& a, hipStream_t s)
@@ -1145,7 +1178,7 @@ hipError_t launch_air_mask(const BatchAirArgs& a, hipStream_t s)
 
 hipError_t launch_air_reset(const BatchAirArgs& a, hipStream_t s)
 {
-    if (a.n_boxes <= 0 || !a.next_done)
+    if (a.n_boxes <= 0 || !a.done)
         return hipSuccess;
     hipLaunchKernelGGL(air_reset_kernel, dim3(unsigned((a.n_boxes + 255) / 256)), dim3(256), 0, s,
                        a);

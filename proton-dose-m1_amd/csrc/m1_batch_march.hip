@@ -244,11 +244,14 @@ struct m1_batch {
     uint8_t* d_air = nullptr;         // per tile: quasi-steady tiles to relax
     int* d_air_list = nullptr;
     int* d_air_count = nullptr;
-    float* d_relax_tmp = nullptr;     // 4 x cells: a relaxation sweep's result
+    // 4 x cells each: the sweeps' results, alternately (zero outside the
+    // tiles listed for relaxation, so that a sweep reads the zero state of
+    // the quasi-steady cells of tiles out of the live set from them)
+    float* d_relax_tmp[2] = {nullptr, nullptr};
     double qs_tol = 0.0;              // per-box convergence of the sweeps (0: fixed count)
-    uint8_t* d_relax_done = nullptr;  // 2 x boxes: converged before the sweep / after it
-    unsigned* d_relax_change = nullptr;  // 2 x boxes: a sweep's largest change (float bits)
-    long tiles_with_mat[MAXMAT] = {};  // tiles of the boxes holding material m
+    uint8_t* d_relax_done = nullptr;  // 2 x boxes: converged before a sweep / after it
+    unsigned* d_relax_change = nullptr;  // 3 x boxes: the sweeps' largest changes (float bits)
+    long tiles_with_mat[MAXMAT] = {};  // tiles holding material m
     int progress_every = 0;  // m1_batch_set_progress: a status line every N device steps
     // device-side energy stepping (m1_batch_set_device_steps)
     bool device_steps = false;
@@ -321,11 +324,12 @@ void m1_batch::free_problem()
             (void)hipFree(p);
     for (void* p : {(void*)d_mats, (void*)d_tables, (void*)d_bc, (void*)d_bst, (void*)d_ctr,
                     (void*)d_relax32, (void*)d_air, (void*)d_air_list, (void*)d_air_count,
-                    (void*)d_relax_tmp, (void*)d_relax_done, (void*)d_relax_change})
+                    (void*)d_relax_tmp[0], (void*)d_relax_tmp[1], (void*)d_relax_done,
+                    (void*)d_relax_change})
         if (p)
             (void)hipFree(p);
     d_relax32 = nullptr;
-    d_relax_tmp = nullptr;
+    d_relax_tmp[0] = d_relax_tmp[1] = nullptr;
     d_relax_done = nullptr;
     d_relax_change = nullptr;
     d_air = nullptr;
@@ -908,9 +912,13 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
                     dev_alloc(&c->d_air, T, "hipMalloc quasi-steady tiles");
                     dev_alloc(&c->d_air_list, T, "hipMalloc quasi-steady tile list");
                     dev_alloc(&c->d_air_count, LIST_HEADER_ALLOC, "hipMalloc tile count");
-                    dev_alloc(&c->d_relax_tmp, 4 * N, "hipMalloc relaxation state");
+                    for (float*& r : c->d_relax_tmp) {
+                        dev_alloc(&r, 4 * N, "hipMalloc relaxation state");
+                        hip_check(hipMemsetAsync(r, 0, 4 * N * sizeof(float), s),
+                                  "relaxation state clear");
+                    }
                     dev_alloc(&c->d_relax_done, 2 * nb, "hipMalloc relaxation flags");
-                    dev_alloc(&c->d_relax_change, 2 * nb, "hipMalloc relaxation changes");
+                    dev_alloc(&c->d_relax_change, 3 * nb, "hipMalloc relaxation changes");
                 }
                 std::vector<StepCoef32> rc(nb);
                 for (size_t b = 0; b < nb; ++b) {
@@ -1092,6 +1100,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             }
             BatchZeroArgs za{c->d_box,         c->d_tile_box, int(T), c->d_flags[k],
                              c->d_flags[1 - k], c->U,         c->V1,  c->V2, c->W1, c->W2};
+            if (qs) {
+                za.R1 = c->d_relax_tmp[0];
+                za.R2 = c->d_relax_tmp[1];
+            }
             Timed t(c->prof, s, K_TILE_ZERO);
             hip_check(c->K->zero(za, s), "tile zero kernel (batch)");
         };
@@ -1180,17 +1192,24 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             if (c->qs_mask >> m & 1u)
                 air_bound += c->tiles_with_mat[m];
         const int air_est = int(std::max(8L, std::min(long(T), (air_bound + 7) / 8 * 8)));
-        // With a tolerance, a box stops sweeping once a sweep changes none of
-        // its quasi-steady cells by more than it (the float state is scaled
-        // to the beamlet's peak inflow fluence, so the bound is relative to
-        // each beamlet): the flags and changes alternate between two halves.
+        // Sweep j reads the quasi-steady cells from the output of sweep j - 1
+        // (the first from the state) and writes them to d_relax_tmp[j % 2];
+        // the last sweep's output is committed to the state.  With a
+        // tolerance, a box stops sweeping once a sweep changes none of its
+        // quasi-steady cells by more than it (the float state is scaled to the
+        // beamlet's peak inflow fluence, so the bound is relative to each
+        // beamlet): its tiles then carry their cells through the remaining
+        // sweeps.  The flags alternate between two halves, the changes rotate
+        // through three (read, filled, cleared).
         const bool qs_conv = c->qs_tol > 0.0;
         auto relax_air = [&](int k, float* state, int bnd_stage, bool build_list) {
+            const int last = (c->qs_iters - 1) % 2;
             BatchAirArgs aa{c->d_box, c->d_step, c->d_tile_box, int(T), c->d_flags[k],
                             c->d_tile_mats, c->qs_mask, c->d_air, c->d_air_list, c->d_air_count,
-                            c->ids, c->d_relax_tmp, state};
+                            c->ids, c->d_relax_tmp[last], state};
             aa.n_boxes = int(B);
-            aa.tol = float(c->qs_tol);
+            aa.done = c->d_relax_done;
+            aa.change = c->d_relax_change;
             if (build_list) {
                 hip_check(c->K->air_mask(aa, s), "quasi-steady tile mask");
                 hip_check(c->K->compact(c->d_air, int(T), c->d_air_list, c->d_air_count, s),
@@ -1200,12 +1219,10 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             ra.list = c->d_air_list;
             ra.count = c->d_air_count;
             ra.queue = nullptr;
-            ra.strided = air_est;
-            ra.n_list = air_est + overflow_wgs(air_est);
+            ra.strided = air_est;  // at least the tiles holding the materials: no overflow
+            ra.n_list = air_est;
             ra.src = state;
-            ra.out = c->d_relax_tmp;
             ra.U = state;
-            ra.V1 = nullptr;
             ra.win = nullptr;
             ra.wout = nullptr;
             ra.coef = c->d_relax32;
@@ -1215,23 +1232,23 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
             ra.tile_face = nullptr;
             ra.judge = nullptr;
             ra.bnd_stage = bnd_stage;
-            if (qs_conv) {
-                aa.next_done = c->d_relax_done;
-                aa.next_change = c->d_relax_change;
+            ra.relax_tol = float(c->qs_tol);
+            Timed t(c->prof, s, K_RELAX);
+            if (qs_conv)
                 hip_check(c->K->air_reset(aa, s), "relaxation flag reset");
-            }
             for (int it = 0; it < c->qs_iters; ++it) {
+                ra.V1 = it == 0 ? nullptr : c->d_relax_tmp[(it - 1) % 2];
+                ra.out = c->d_relax_tmp[it % 2];
                 if (qs_conv) {
-                    const int p = it & 1;
-                    aa.done = ra.relax_done = c->d_relax_done + p * B;
-                    aa.change = ra.relax_change = c->d_relax_change + p * B;
-                    aa.next_done = c->d_relax_done + (1 - p) * B;
-                    aa.next_change = c->d_relax_change + (1 - p) * B;
+                    ra.relax_done_in = c->d_relax_done + (it % 2) * B;
+                    ra.relax_done_out = c->d_relax_done + ((it + 1) % 2) * B;
+                    ra.relax_change_prev = c->d_relax_change + ((it + 2) % 3) * B;
+                    ra.relax_change_cur = c->d_relax_change + (it % 3) * B;
+                    ra.relax_change_clear = c->d_relax_change + ((it + 1) % 3) * B;
                 }
-                Timed t(c->prof, s, K_RELAX);
                 hip_check(c->K->stage(STAGE_RELAX, false, ra, s), "relaxation kernel");
-                hip_check(c->K->air_commit(aa, std::min(air_est, 16384), s), "relaxation commit");
             }
+            hip_check(c->K->air_commit(aa, std::min(air_est, 16384), s), "relaxation commit");
         };
         int graph_est = 8;  // the estimate a device-stepped graph is captured with
         int graph_set[2] = {0, 0};  // the estimate each graph's stage grids hold

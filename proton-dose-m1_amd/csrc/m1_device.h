@@ -301,11 +301,22 @@ struct BatchStageArgs {
     // one this stage forms for the next (wout); nullable.
     const float* win;
     float* wout;
-    // Relaxation sweeps (STAGE_RELAX; nullable): boxes flagged in
-    // relax_done skip the sweep, the others fold their quasi-steady cells'
-    // largest change into relax_change[box] (ordered float bits).
-    const uint8_t* relax_done;
-    unsigned* relax_change;
+    // Relaxation sweeps (STAGE_RELAX): the quasi-steady cells are read from
+    // V1 (the previous sweep's output; null: from src) and written to out,
+    // the other cells read from src.  With convergence flags (relax_done_in
+    // not null), a box is done for this sweep when it was done or the
+    // previous sweep's largest change of its quasi-steady cells
+    // (relax_change_prev, ordered float bits) is below relax_tol: its tiles
+    // copy their quasi-steady cells from V1 to out; the others fold this
+    // sweep's largest change into relax_change_cur.  Every workgroup of a box
+    // also writes the box's flag for the next sweep (relax_done_out) and
+    // clears its slot of the sweep after it (relax_change_clear).
+    const uint8_t* relax_done_in;
+    uint8_t* relax_done_out;
+    const unsigned* relax_change_prev;
+    unsigned* relax_change_cur;
+    unsigned* relax_change_clear;
+    float relax_tol;
 };
 
 // Per list: its length followed by one set of XCD queue counters per stage
@@ -437,16 +448,12 @@ struct BatchAirArgs {
     const uint8_t* ids;
     const float* from;
     float* to;
-    // Convergence of the sweeps per box (nullable: a fixed sweep count).  The
-    // commit of a sweep skips boxes flagged in done, and sets next_done of a
-    // box when it was done or the sweep's largest change (change, ordered
-    // float bits) fell below tol; it clears next_change for the next sweep.
+    // reset of the sweeps' convergence flags: done[b] = 0 (the flags the
+    // first sweep reads), change[b] = 0 (the slot it fills) and
+    // change[2 n_boxes + b] = +inf (the slot it reads as the change before it)
     int n_boxes;
-    const uint8_t* done;
-    uint8_t* next_done;
-    const unsigned* change;
-    unsigned* next_change;
-    float tol;
+    uint8_t* done;
+    unsigned* change;
 };
 
 // Judge: per box, the largest |full - skip| dose increment and the largest
@@ -497,6 +504,8 @@ struct BatchZeroArgs {
     float* V2;
     float* W1;  // the stage inputs formed by stages 0 and 1 (nullable)
     float* W2;
+    float* R1;  // the quasi-steady relaxation's sweep outputs (nullable)
+    float* R2;
 };
 
 // Launchers of the batched kernels compiled for one tile shape.
@@ -531,7 +540,7 @@ struct BatchKernels {
     // estimate, each taking entries grid-stride)
     hipError_t (*air_mask)(const BatchAirArgs& a, hipStream_t s);
     hipError_t (*air_commit)(const BatchAirArgs& a, int grid, hipStream_t s);
-    // before a series of sweeps: done[b] = 0, change[b] = 0 for every box
+    // before a series of sweeps: the convergence flags' reset (see BatchAirArgs)
     hipError_t (*air_reset)(const BatchAirArgs& a, hipStream_t s);
 };
 

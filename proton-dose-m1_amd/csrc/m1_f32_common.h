@@ -453,10 +453,17 @@ __device__ __forceinline__ void load_state(const TileView& a, const StepCoef32& 
         u[3] = a.win[i + 3 * N];
         return;
     }
-    u[0] = a.src[i];
-    u[1] = a.src[i + N];
-    u[2] = a.src[i + 2 * N];
-    u[3] = a.src[i + 3 * N];
+    const float* src = a.src;
+    if constexpr (KIND == STAGE_RELAX) {
+        // quasi-steady cells (a nonzero pseudo-time CFL number) from the
+        // previous sweep's output, when there is one
+        if (a.V1 && c.dEm[a.ids[i]] != 0.f)
+            src = a.V1;
+    }
+    u[0] = src[i];
+    u[1] = src[i + N];
+    u[2] = src[i + 2 * N];
+    u[3] = src[i + 3 * N];
     if constexpr (KIND == STAGE1 || KIND == STAGE2) {
         const int m = a.ids[i];
         const float gp = KIND == STAGE1 ? c.g1p[m] : c.g2p[m];
@@ -921,7 +928,8 @@ __device__ __forceinline__ void stage_tile(const TileView& a)
                 // a pseudo-time step towards div F = 0 of a quasi-steady cell,
                 // dtau = cfl / (sum over the axes of its speed / cell size)
                 if (dEr != 0.f) {
-                    const double sp = cfl_cell(a.src, i, N, 1.0, a.g, wave, eeps, false, 0.0, 1.0);
+                    const double sp = cfl_cell(a.V1 ? a.V1 : a.src, i, N, 1.0, a.g, wave, eeps,
+                                               false, 0.0, 1.0);
                     const float dtau = sp > 0.0 ? float(double(dEr) / sp) : 0.f;
                     float u[4];
 #pragma unroll
