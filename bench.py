@@ -775,6 +775,20 @@ def air_validation(args):
     }
 
 
+def combined_sample_picks(phantom, beamlets, boxes, air_free, n):
+    """Beamlets of the combined workload checked against the FP64
+    moving-domain march: n energy quantiles of the air-free beamlets that
+    stop inside the phantom (box shallower than it).  Past the far face the
+    moving domain of an exiting beam widens by ~50 cells and reaches air (a
+    227 MeV beamlet's FP64 march drops to ~4e-4 MeV steps at 85 MeV there),
+    hours per beamlet."""
+    stopping = [b for b in air_free if boxes[b][4] < phantom.shape[2]]
+    if n <= 0 or not stopping:
+        return []
+    order = sorted(stopping, key=lambda b: beamlets[b].energy)
+    return sorted({order[int(round(q * (len(order) - 1)))] for q in np.linspace(0.0, 1.0, n)})
+
+
 def run_combined_quasi_steady(args):
     """The combined five-material workload in full: all its beamlets in one
     batch with the air quasi-steady (held through each stage and relaxed
@@ -841,14 +855,8 @@ def run_combined_quasi_steady(args):
 
     # FP64 moving-domain samples of air-free beamlets, at the batch's maximum step
     rows = []
-    # only beamlets that stop inside the phantom: past the far face the moving
-    # domain of an exiting beam widens by ~50 cells and reaches air (a 227 MeV
-    # beamlet's FP64 march drops to ~4e-4 MeV steps at 85 MeV there)
-    stopping = [b for b in air_free if boxes[b][4] < phantom.shape[2]]
-    if args.fp64_sample > 0 and stopping:
-        order = sorted(stopping, key=lambda b: beamlets[b].energy)
-        picks = sorted({order[int(round(q * (len(order) - 1)))]
-                        for q in np.linspace(0.0, 1.0, args.fp64_sample)})
+    picks = combined_sample_picks(phantom, beamlets, boxes, air_free, args.fp64_sample)
+    if picks:
         cfg64 = dataclasses.replace(cfg, numerics=dataclasses.replace(cfg.numerics,
                                                                       vacuum_aware_cfl=True))
         num64 = _native.numerics_struct(cfg64, _native.FP64)

@@ -9,7 +9,9 @@
   * the wall-clock budget cuts the warm-up and timed batches so that the
     driver's `--steps 20 --warmup 5` finishes in time;
   * the CPU baseline (the oracle on a few energy steps of a box) reports the
-    all-thread and the one-thread rate with the host CPU model.
+    all-thread and the one-thread rate with the host CPU model;
+  * the combined workload's FP64 samples are air-free beamlets that stop
+    inside the phantom.
 """
 
 import subprocess
@@ -124,3 +126,25 @@ def test_cpu_baseline_of_a_batch_beamlet():
     assert r["single_thread"]["cell_steps_per_s"] > 0
     assert "first 1 energy steps" in r["single_thread"]["sample"]
     assert r["cpu_model"] is None or isinstance(r["cpu_model"], str)
+
+
+def test_combined_samples_are_air_free_beamlets_that_stop_inside():
+    """The combined workload's FP64 samples: air-free beamlets whose boxes end
+    before the far face (an exiting beam's moving domain reaches air past
+    it), spread over the energies, among them one above 150 MeV."""
+    from proton_solver.batch import (air_free_beamlets, batch_workload_config, beamlet_boxes,
+                                     combined_workload)
+
+    phantom, beamlets = combined_workload()
+    n = batch_workload_config().numerics
+    boxes = beamlet_boxes(phantom, beamlets, n.energy_min, n.energy_window_sigma)
+    air_free = air_free_beamlets(phantom, beamlets, boxes)
+    picks = bench.combined_sample_picks(phantom, beamlets, boxes, air_free, 3)
+    assert len(picks) == 3
+    assert set(picks) <= set(air_free)
+    assert all(boxes[b][4] < phantom.shape[2] for b in picks)
+    # some air-free beamlets exit through the far face and are left out
+    assert any(boxes[b][4] == phantom.shape[2] for b in air_free)
+    energies = sorted(beamlets[b].energy for b in picks)
+    assert energies[0] < 80.0 and energies[-1] > 150.0
+    assert bench.combined_sample_picks(phantom, beamlets, boxes, air_free, 0) == []
