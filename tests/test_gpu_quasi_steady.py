@@ -12,7 +12,8 @@ against that plain FP64 march (bitwise the CPU oracle): maximum dose within
 march's own energy grid is ~1e-3 MeV here; at 0.25 MeV the water alone
 differs from a converged energy grid by ~0.5 % in maximum dose).  The same
 with the sweeps stopping per beamlet at a tolerance
-(m1_batch_set_quasi_steady_tolerance).  Also the argument checks.
+(m1_batch_set_quasi_steady_tolerance), the convergence machinery bitwise
+neutral when no box converges, and the argument checks.
 """
 
 import dataclasses
@@ -78,6 +79,19 @@ def test_quasi_steady_air_against_the_plain_fp64_march(plain_fp64, sweeps, toler
     assert gamma >= 99.5
     # the dose before the pocket (z < 9 mm) is the water march's
     assert np.abs(got[:, :, :9] - ref[:, :, :9]).max() <= 2e-3 * ref.max()
+
+
+def test_convergence_flags_keep_the_fixed_sweeps():
+    """With a tolerance no change can fall below (1e-30: only boxes whose air
+    cells stop changing exactly are done), the convergence machinery --
+    per-box flags and changes, the copy of a done box into both scratch
+    states -- leaves the march bitwise that of the fixed sweep count."""
+    _, steps0, fixed = _quasi_steady(32, 0.0)
+    _, steps1, flagged = _quasi_steady(32, 1e-30)
+    say(f"fixed sweeps against flags at 1e-30: steps {steps0} / {steps1}, max |dD| "
+        f"{float(np.abs(fixed - flagged).max()):.3e}")
+    assert steps0 == steps1
+    assert np.array_equal(fixed, flagged)
 
 
 def test_quasi_steady_argument_checks():
