@@ -13,7 +13,9 @@ march's own energy grid is ~1e-3 MeV here; at 0.25 MeV the water alone
 differs from a converged energy grid by ~0.5 % in maximum dose).  The same
 with the sweeps stopping per beamlet at a tolerance
 (m1_batch_set_quasi_steady_tolerance), the convergence machinery bitwise
-neutral when no box converges, and the argument checks.
+neutral when no box converges, a beamlet of the combined five-material
+workload with air beside its axis against the plain FP64 march of its box,
+and the argument checks.
 """
 
 import dataclasses
@@ -24,7 +26,8 @@ import pytest
 from proton_solver import Material, _native, metrics
 from proton_solver.autotune import set_mode
 from proton_solver.batch import (
-    Beamlet, Ellipsoid, batch_workload_config, ellipsoid_phantom, full_problem, prepare_batch,
+    Beamlet, Ellipsoid, batch_workload_config, beamlet_boxes, box_problem, combined_workload,
+    ellipsoid_phantom, full_problem, prepare_batch,
 )
 from test_gpu_parity import say
 
@@ -92,6 +95,56 @@ def test_convergence_flags_keep_the_fixed_sweeps():
         f"{float(np.abs(fixed - flagged).max()):.3e}")
     assert steps0 == steps1
     assert np.array_equal(fixed, flagged)
+
+
+def test_combined_workload_beamlet_near_air_against_the_plain_fp64_march():
+    """The lowest-energy beamlet of the combined workload with air within
+    5 mm of its axis in its box (beamlet 503, 78 MeV, 12,220 air cells):
+    batched with the air quasi-steady (at most 128 sweeps, tolerance 1e-5,
+    0.1 MeV maximum step) against the plain FP64 march of its box, which
+    takes air-bound steps."""
+    phantom, beamlets = combined_workload()
+    cfg = batch_workload_config()
+    cfg = dataclasses.replace(cfg, numerics=dataclasses.replace(cfg.numerics,
+                                                                max_energy_step=0.1))
+    n = cfg.numerics
+    boxes = beamlet_boxes(phantom, beamlets, n.energy_min, n.energy_window_sigma)
+    air = phantom.materials.index(Material.AIR)
+    px, py, _ = phantom.shape
+    dx = phantom.spacing[0]
+
+    def air_distance(b):
+        x0, y0, nx, ny, nz = boxes[b]
+        cells = np.argwhere(phantom.ids[x0:x0 + nx, y0:y0 + ny, :nz] == air)
+        if not len(cells):
+            return np.inf
+        ix = np.floor((beamlets[b].center[0] + 0.5 * px * dx) / dx) - x0
+        iy = np.floor((beamlets[b].center[1] + 0.5 * py * dx) / dx) - y0
+        return float(np.hypot(cells[:, 0] - ix, cells[:, 1] - iy).min())
+
+    near = [b for b in np.argsort([bl.energy for bl in beamlets]) if air_distance(b) < 5.0]
+    b = int(near[0])
+    handle, _ = prepare_batch(phantom, [beamlets[b]], cfg, boxes=[boxes[b]], tile_shape=2)
+    set_mode(handle, "device")
+    handle.set_quasi_steady(1 << air, 128, 0.3)
+    handle.set_quasi_steady_tolerance(1e-5)
+    _, steps = handle.march(_native.numerics_struct(cfg), want_dose=False)
+    got = handle.beamlet_dose(0)
+    handle.close()
+    ctx = _native.Context(0)
+    ctx.set_problem(*box_problem(phantom, boxes[b], beamlets[b]))
+    ctx.set_state(None)
+    say(f"plain FP64 march of combined beamlet {b} ({beamlets[b].energy:.1f} MeV) on its box")
+    E64, _ = ctx.march(_native.numerics_struct(cfg, _native.FP64))
+    ref = ctx.get_dose()
+    err = metrics.maximum_dose_relative_difference(got, ref)
+    gamma = metrics.gamma_pass_rate(got, ref, phantom.spacing, 1.0, 1.0, 10.0, device=0)
+    say(f"combined beamlet {b}: {int(steps[0])} steps against {len(E64) - 1}; max-dose error "
+        f"{err:+.2e}, max |dD| / max {metrics.max_abs_difference_over_max(got, ref):.2e}, "
+        f"gamma {gamma:.3f} %")
+    assert len(E64) - 1 > 50 * int(steps[0])
+    assert abs(err) <= 5e-3
+    assert gamma >= 99.5
 
 
 def test_quasi_steady_argument_checks():
