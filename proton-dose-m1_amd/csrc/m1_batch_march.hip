@@ -1184,9 +1184,8 @@ int m1_batch_march(m1_batch* c, const m1_numerics* num, double* dose, int32_t* n
         // Quasi-steady materials: after each stage, qs_iters Jacobi sweeps of
         // a pseudo-time step towards div F = 0 on their cells of the next
         // stage's stencil input (W1, W2, then U; the other cells fixed), over
-        // the live tiles holding them: a sweep's result goes to a scratch
-        // state, then back.  The quasi-steady cells thus follow the other
-        // cells at every stage (no lag of a step).
+        // the live tiles of active boxes holding them.  The quasi-steady cells
+        // thus follow the other cells at every stage (no lag of a step).
         long air_bound = 0;
         for (int m = 0; m < MAXMAT; ++m)
             if (c->qs_mask >> m & 1u)
