@@ -62,8 +62,14 @@ def sample_beamlets(beamlets, n):
     return [beamlets[i] for i in picks]
 
 
+# Graph replay is taken when it is within this fraction of the fastest launch:
+# the sample's timings scatter by 1-2 % between runs, and on the whole
+# 256-beamlet batch graph replay and the launched steps are within 0.3 %.
+PREFER_WITHIN = 0.02
+
+
 def autotune(phantom, beamlets, config, sample=12, shapes=None, modes=MODES, device=None,
-             log=None, setup=None, prefer=("device_graphs",), prefer_within=0.01):
+             log=None, setup=None, prefer=("device_graphs",), prefer_within=PREFER_WITHIN):
     """Fastest (tile shape, launch) on a sample of the batch; returns a Choice
     whose table lists every timed candidate.  setup(handle), when given, is
     applied to every candidate batch before it is timed (e.g. the tile-skip
